@@ -1,0 +1,244 @@
+// dense.hip -- the n0 x n0 seed block in FP64 (gfx950).
+//
+// Reference: graphgp/refine.py:59-76 (generate_dense: chol(K) @ xi), :186-195
+// (generate_dense_inv: solve(L, values)), :243-248 (generate_dense_logdet:
+// slogdet(K)/2 = sum log diag L).  The seed block is always factorised in FP64.
+//
+// Kernels: covariance assembly (one thread per entry, binary-search interp on the FP64
+// table), a right-looking blocked Cholesky with 64-wide panels (panel kernel: the
+// diagonal block is factorised in LDS by one wavefront and every row block of the
+// panel is solved against it; update kernel: 64x64 LDS tiles of the trailing lower
+// triangle), a wavefront-per-row triangular matvec, a blocked forward substitution in
+// one workgroup, and a fixed-order log-diagonal sum.
+#include "common.hpp"
+#include "dense.hpp"
+
+namespace ggp {
+
+namespace {
+
+constexpr int NB = 64;
+
+__device__ inline double interp_dev(double x, const double* __restrict__ xp, const double* __restrict__ fp, int n) {
+  if (x < xp[0]) return fp[0];
+  if (x > xp[n - 1]) return fp[n - 1];
+  int lo = 0, hi = n;  // upper_bound
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (xp[mid] <= x) lo = mid + 1;
+    else hi = mid;
+  }
+  int i = lo < 1 ? 1 : (lo > n - 1 ? n - 1 : lo);
+  double df = fp[i] - fp[i - 1], dx = xp[i] - xp[i - 1], delta = x - xp[i - 1];
+  if (fabs(dx) <= 4.930380657631324e-32) return fp[i - 1];
+  return fp[i - 1] + (delta / dx) * df;
+}
+
+__global__ void dense_cov_kernel(const void* __restrict__ pts, int pts_f64, int d, int64_t n,
+                                 const double* __restrict__ bins, const double* __restrict__ vals, int nb,
+                                 double* __restrict__ A) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * n) return;
+  int64_t i = e / n, j = e - i * n;
+  double s = 0.0;
+  for (int t = 0; t < d; ++t) {
+    double a = pts_f64 ? ((const double*)pts)[i * d + t] : (double)((const float*)pts)[i * d + t];
+    double b = pts_f64 ? ((const double*)pts)[j * d + t] : (double)((const float*)pts)[j * d + t];
+    double u = a - b;
+    s += u * u;
+  }
+  A[e] = interp_dev(sqrt(s), bins, vals, nb);
+}
+
+// Factorise the diagonal block D (nbj x nbj, in LDS, lower) with one wavefront.
+__device__ inline void chol_lds(double (*Ds)[NB + 1], int nbj) {
+  const int t = threadIdx.x;
+  for (int c = 0; c < nbj; ++c) {
+    const double dc = sqrt(Ds[c][c]);
+    __syncthreads();
+    if (t == c) Ds[c][c] = dc;
+    if (t > c && t < nbj) Ds[t][c] /= dc;
+    __syncthreads();
+    if (t > c && t < nbj) {
+      const double lt = Ds[t][c];
+      for (int m = c + 1; m <= t; ++m) Ds[t][m] -= lt * Ds[m][c];
+    }
+    __syncthreads();
+  }
+}
+
+// blockIdx.x = row block offset from the diagonal block jb (0 = the diagonal block).
+__global__ __launch_bounds__(64) void panel_kernel(double* __restrict__ A, int64_t n, int jb) {
+  __shared__ double Ds[NB][NB + 1];
+  const int64_t j0 = (int64_t)jb * NB;
+  const int nbj = (int)min<int64_t>(NB, n - j0);
+  const int t = threadIdx.x;
+  for (int r = 0; r < nbj; ++r)
+    if (t <= r) Ds[r][t] = A[(j0 + r) * n + j0 + t];
+  __syncthreads();
+  chol_lds(Ds, nbj);
+  const int ib = jb + blockIdx.x;
+  if (blockIdx.x == 0) {
+    for (int r = 0; r < nbj; ++r) {
+      if (t <= r) A[(j0 + r) * n + j0 + t] = Ds[r][t];
+      else if (t < nbj) A[(j0 + r) * n + j0 + t] = 0.0;  // clear the upper part of the block
+    }
+    return;
+  }
+  const int64_t i = (int64_t)ib * NB + t;
+  if (i >= n) return;
+  double x[NB];
+  double* row = A + i * n + j0;
+#pragma unroll
+  for (int c = 0; c < NB; ++c) x[c] = c < nbj ? row[c] : 0.0;
+  // x L^T = a  ->  x_c = (a_c - sum_{m<c} x_m L_cm) / L_cc
+#pragma unroll
+  for (int c = 0; c < NB; ++c) {
+    if (c < nbj) {
+      double s = x[c];
+#pragma unroll
+      for (int m = 0; m < c; ++m) s -= x[m] * Ds[c][m];
+      x[c] = s / Ds[c][c];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < NB; ++c)
+    if (c < nbj) row[c] = x[c];
+}
+
+// Trailing update A[ib][mb] -= L[ib][jb] L[mb][jb]^T for jb < mb <= ib (lower blocks).
+__global__ __launch_bounds__(256) void update_kernel(double* __restrict__ A, int64_t n, int jb, int nrem) {
+  __shared__ double Li[NB][NB + 1];
+  __shared__ double Lm[NB][NB + 1];
+  // decode blockIdx.x -> (a, b) with 0 <= b <= a < nrem
+  int e = blockIdx.x;
+  int aa = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+  while ((aa + 1) * (aa + 2) / 2 <= e) ++aa;
+  while (aa * (aa + 1) / 2 > e) --aa;
+  const int bb = e - aa * (aa + 1) / 2;
+  const int ib = jb + 1 + aa, mb = jb + 1 + bb;
+  const int64_t j0 = (int64_t)jb * NB, i0 = (int64_t)ib * NB, m0 = (int64_t)mb * NB;
+  const int nbj = (int)min<int64_t>(NB, n - j0);
+  const int t = threadIdx.x;
+  for (int q = t; q < NB * NB; q += 256) {
+    const int r = q / NB, c = q % NB;
+    Li[r][c] = (i0 + r < n && c < nbj) ? A[(i0 + r) * n + j0 + c] : 0.0;
+    Lm[r][c] = (m0 + r < n && c < nbj) ? A[(m0 + r) * n + j0 + c] : 0.0;
+  }
+  __syncthreads();
+  const int tr = (t / 16) * 4, tc = (t % 16) * 4;
+  double acc[4][4] = {};
+  for (int c = 0; c < nbj; ++c) {
+    double li[4], lm[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) { li[u] = Li[tr + u][c]; lm[u] = Lm[tc + u][c]; }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[u][v] += li[u] * lm[v];
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t r = i0 + tr + u, c = m0 + tc + v;
+      if (r < n && c < n && c <= r) A[r * n + c] -= acc[u][v];
+    }
+}
+
+// values[i] = sum_{j<=i} L[i][j] xs[j]; one wavefront per row.
+__global__ __launch_bounds__(256) void trmv_kernel(const double* __restrict__ L, int64_t n,
+                                                   const double* __restrict__ xs, double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int64_t j = lane; j <= i; j += 64) s += L[i * n + j] * xs[j];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (lane == 0) out[i] = s;
+}
+
+// Solve L x = v for x (in place on v); one workgroup of 1024 threads, 64-row blocks.
+__global__ __launch_bounds__(1024) void trsv_kernel(const double* __restrict__ L, int64_t n, double* __restrict__ v) {
+  __shared__ double xb[NB];
+  const int t = threadIdx.x;
+  for (int64_t b0 = 0; b0 < n; b0 += NB) {
+    const int nbb = (int)min<int64_t>(NB, n - b0);
+    if (t < 64) {
+      // sequential solve of the diagonal block by one wavefront (lane c finalises x_c)
+      double vi = t < nbb ? v[b0 + t] : 0.0;
+      for (int c = 0; c < nbb; ++c) {
+        double xc = __shfl(vi, c, 64) / L[(b0 + c) * n + b0 + c];
+        if (t == c) vi = xc;
+        if (t > c && t < nbb) vi -= L[(b0 + t) * n + b0 + c] * xc;
+        if (t == c) xb[c] = xc;
+      }
+      if (t < nbb) v[b0 + t] = vi;
+    }
+    __syncthreads();
+    for (int64_t i = b0 + nbb + t; i < n; i += 1024) {
+      double s = v[i];
+      const double* row = L + i * n + b0;
+      for (int c = 0; c < nbb; ++c) s -= row[c] * xb[c];
+      v[i] = s;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(1024) void logdiag_kernel(const double* __restrict__ L, int64_t n, double* out) {
+  __shared__ double red[16];
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 1024) s += log(L[i * n + i]);
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tsum = 0.0;
+    for (int w = 0; w < 16; ++w) tsum += red[w];
+    *out = tsum;
+  }
+}
+
+}  // namespace
+
+size_t dense_workspace(int64_t n0) { return (size_t)(n0 * n0) * sizeof(double) + 256; }
+
+int dense_factor(const void* pts, int pts_f64, int d, int64_t n0, const double* bins, const double* vals, int nb,
+                 double* A, hipStream_t s) {
+  if (n0 <= 0) return kOk;
+  hipLaunchKernelGGL(dense_cov_kernel, dim3(grid_for(n0 * n0, 256)), dim3(256), 0, s, pts, pts_f64, d, n0, bins,
+                     vals, nb, A);
+  GGP_LAUNCH_CHECK();
+  const int nblk = (int)((n0 + NB - 1) / NB);
+  for (int jb = 0; jb < nblk; ++jb) {
+    hipLaunchKernelGGL(panel_kernel, dim3(nblk - jb), dim3(64), 0, s, A, n0, jb);
+    const int nrem = nblk - jb - 1;
+    if (nrem > 0)
+      hipLaunchKernelGGL(update_kernel, dim3(nrem * (nrem + 1) / 2), dim3(256), 0, s, A, n0, jb, nrem);
+    GGP_LAUNCH_CHECK();
+  }
+  return kOk;
+}
+
+int dense_apply(const double* L, int64_t n0, const double* xs, double* out, hipStream_t s) {
+  if (n0 <= 0) return kOk;
+  hipLaunchKernelGGL(trmv_kernel, dim3(grid_for(n0, 4)), dim3(256), 0, s, L, n0, xs, out);
+  GGP_LAUNCH_CHECK();
+  return kOk;
+}
+
+int dense_solve(const double* L, int64_t n0, double* v, hipStream_t s) {
+  if (n0 <= 0) return kOk;
+  hipLaunchKernelGGL(trsv_kernel, dim3(1), dim3(1024), 0, s, L, n0, v);
+  GGP_LAUNCH_CHECK();
+  return kOk;
+}
+
+int dense_logdiag(const double* L, int64_t n0, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(logdiag_kernel, dim3(1), dim3(1024), 0, s, L, n0, out);
+  GGP_LAUNCH_CHECK();
+  return kOk;
+}
+
+}  // namespace ggp

@@ -1,0 +1,13 @@
+// dense.hpp -- FP64 seed-block routines (dense.hip).
+#pragma once
+#include "common.hpp"
+
+namespace ggp {
+size_t dense_workspace(int64_t n0);
+// A <- chol(K(points[0:n0])) (lower, row-major n0 x n0).
+int dense_factor(const void* pts, int pts_f64, int d, int64_t n0, const double* bins, const double* vals, int nb,
+                 double* A, hipStream_t s);
+int dense_apply(const double* L, int64_t n0, const double* xs, double* out, hipStream_t s);  // out = L xs
+int dense_solve(const double* L, int64_t n0, double* v, hipStream_t s);                     // v <- L^{-1} v
+int dense_logdiag(const double* L, int64_t n0, double* out, hipStream_t s);                 // sum log L_ii
+}  // namespace ggp

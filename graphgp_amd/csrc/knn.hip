@@ -1,0 +1,273 @@
+// knn.hip -- preceding k-nearest-neighbour search over the level-interleaved k-d tree.
+//
+// Reference: graphgp/tree.py:41-150.  One thread per query runs the reference's
+// stack-free parent-pointer traversal (tree.py:108-150) exactly: the node is offered
+// to the candidate list on first arrival from its parent, the near child is the left
+// one iff q[dim] - p[dim] < 0, a far child is entered iff it exists (< max_index) and
+// (q[dim] - p[dim])^2 <= r^2.  The candidate list keeps k slots (distance^2, id),
+// initialised to (+inf, -1); a candidate replaces the FIRST slot holding the maximum
+// iff it is strictly closer (tree.py:81-95).  The result is sorted by
+// (sqrt(distance^2), id) (tree.py:102-104).  Every comparison uses the same separately
+// rounded arithmetic as the reference (this file is built with -ffp-contract=off), so
+// indices match bit for bit, ties included.
+//
+// MI355X mapping: the k slots live in VGPRs (k <= 64, compile-time slot count KMAX with
+// slots >= k pinned at -inf so they never hold the maximum); the final order is a
+// register bitonic network.  Larger k uses slots in a caller workspace.
+#include "common.hpp"
+#include "graphgp_amd_internal.hpp"
+
+namespace ggp {
+
+namespace {
+
+template <typename T, int D>
+struct Query {
+  T c[D];
+  __device__ inline T get(int dim) const {
+    T v = c[0];
+#pragma unroll
+    for (int j = 1; j < D; ++j)
+      if (dim == j) v = c[j];
+    return v;
+  }
+};
+
+template <typename T, int D>
+__device__ inline T dist2(const Query<T, D>& q, const T* __restrict__ p, int d) {
+  T t = q.c[0] - p[0];
+  T s = t * t;
+#pragma unroll
+  for (int j = 1; j < D; ++j) {
+    if (j < d) {
+      T u = q.c[j] - p[j];
+      s = s + u * u;
+    }
+  }
+  return s;
+}
+
+template <typename T>
+__device__ inline bool pair_greater(T da, int32_t ia, T db, int32_t ib) {
+  return da > db || (da == db && ia > ib);
+}
+
+template <typename T, int D, int KMAX>
+__global__ __launch_bounds__(256) void knn_kernel(const T* __restrict__ pts, const int32_t* __restrict__ split,
+                                                  int d, const int32_t* __restrict__ qidx,
+                                                  const int32_t* __restrict__ midx, int64_t q0, int64_t nq, int k,
+                                                  int32_t* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nq) return;
+  const int64_t qi = qidx ? (int64_t)qidx[t] : q0 + t;
+  const int64_t max_index = midx ? (int64_t)midx[t] : qi;
+  Query<T, D> q;
+#pragma unroll
+  for (int j = 0; j < D; ++j) q.c[j] = j < d ? pts[qi * d + j] : T(0);
+
+  const T inf = __builtin_inf();
+  T sd[KMAX];
+  int32_t si[KMAX];
+#pragma unroll
+  for (int j = 0; j < KMAX; ++j) {
+    sd[j] = j < k ? inf : -inf;
+    si[j] = -1;
+  }
+  int amax = 0;
+  T r2 = inf;
+  int64_t cur = 0, prev = -1;
+  while (cur != -1) {
+    const int64_t parent = tree_parent(cur);
+    const T* p = pts + cur * d;
+    if (prev == parent) {
+      const T s = dist2<T, D>(q, p, d);
+      if (s < r2) {
+#pragma unroll
+        for (int j = 0; j < KMAX; ++j)
+          if (j == amax) { sd[j] = s; si[j] = (int32_t)cur; }
+        amax = 0;
+        r2 = sd[0];
+#pragma unroll
+        for (int j = 1; j < KMAX; ++j)
+          if (sd[j] > r2) { r2 = sd[j]; amax = j; }
+      }
+    }
+    const int dim = split[cur];
+    const T sdist = q.get(dim) - p[dim];
+    const int64_t l = tree_left(cur), r = tree_right(cur);
+    const int64_t nearc = sdist < T(0) ? l : r;
+    const int64_t farc = sdist < T(0) ? r : l;
+    const bool far_in = sdist * sdist <= r2;
+    int64_t next;
+    if (prev == nearc || (prev == parent && nearc >= max_index))
+      next = (farc < max_index && far_in) ? farc : parent;
+    else
+      next = prev == parent ? nearc : parent;
+    prev = cur;
+    cur = next;
+  }
+  // final order: (sqrt distance, index); padding slots sort last
+#pragma unroll
+  for (int j = 0; j < KMAX; ++j) {
+    if (j < k) sd[j] = sqrt(sd[j]);
+    else { sd[j] = inf; si[j] = 0x7fffffff; }
+  }
+#pragma unroll
+  for (int size = 2; size <= KMAX; size <<= 1) {
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+#pragma unroll
+      for (int j = 0; j < KMAX; ++j) {
+        const int l = j ^ stride;
+        if (l > j) {
+          const bool asc = (j & size) == 0;
+          const bool gt = pair_greater(sd[j], si[j], sd[l], si[l]);
+          if (gt == asc) {
+            T td = sd[j]; sd[j] = sd[l]; sd[l] = td;
+            int32_t ti = si[j]; si[j] = si[l]; si[l] = ti;
+          }
+        }
+      }
+    }
+  }
+  int32_t* o = out + t * k;
+#pragma unroll
+  for (int j = 0; j < KMAX; ++j)
+    if (j < k) o[j] = si[j];
+}
+
+// Generic k: slots in a per-query workspace region (k*(sizeof(T)+4) bytes per query).
+template <typename T>
+__global__ __launch_bounds__(256) void knn_generic_kernel(const T* __restrict__ pts,
+                                                          const int32_t* __restrict__ split, int d,
+                                                          const int32_t* __restrict__ qidx,
+                                                          const int32_t* __restrict__ midx, int64_t q0, int64_t nq,
+                                                          int k, int32_t* __restrict__ out, T* __restrict__ wsd,
+                                                          int32_t* __restrict__ wsi) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nq) return;
+  const int64_t qi = qidx ? (int64_t)qidx[t] : q0 + t;
+  const int64_t max_index = midx ? (int64_t)midx[t] : qi;
+  const T* q = pts + qi * d;
+  T* sd = wsd + t * k;
+  int32_t* si = wsi + t * k;
+  const T inf = __builtin_inf();
+  for (int j = 0; j < k; ++j) { sd[j] = inf; si[j] = -1; }
+  int amax = 0;
+  T r2 = inf;
+  int64_t cur = 0, prev = -1;
+  while (cur != -1) {
+    const int64_t parent = tree_parent(cur);
+    const T* p = pts + cur * d;
+    if (prev == parent) {
+      T u = q[0] - p[0];
+      T s = u * u;
+      for (int j = 1; j < d; ++j) { u = q[j] - p[j]; s = s + u * u; }
+      if (s < r2) {
+        sd[amax] = s;
+        si[amax] = (int32_t)cur;
+        amax = 0;
+        r2 = sd[0];
+        for (int j = 1; j < k; ++j)
+          if (sd[j] > r2) { r2 = sd[j]; amax = j; }
+      }
+    }
+    const int dim = split[cur];
+    const T sdist = q[dim] - p[dim];
+    const int64_t l = tree_left(cur), r = tree_right(cur);
+    const int64_t nearc = sdist < T(0) ? l : r;
+    const int64_t farc = sdist < T(0) ? r : l;
+    const bool far_in = sdist * sdist <= r2;
+    int64_t next;
+    if (prev == nearc || (prev == parent && nearc >= max_index))
+      next = (farc < max_index && far_in) ? farc : parent;
+    else
+      next = prev == parent ? nearc : parent;
+    prev = cur;
+    cur = next;
+  }
+  for (int j = 0; j < k; ++j) sd[j] = sqrt(sd[j]);
+  // insertion sort by (distance, index)
+  for (int a = 1; a < k; ++a) {
+    T dv = sd[a];
+    int32_t iv = si[a];
+    int b = a - 1;
+    while (b >= 0 && pair_greater(sd[b], si[b], dv, iv)) {
+      sd[b + 1] = sd[b];
+      si[b + 1] = si[b];
+      --b;
+    }
+    sd[b + 1] = dv;
+    si[b + 1] = iv;
+  }
+  for (int j = 0; j < k; ++j) out[t * k + j] = si[j];
+}
+
+template <typename T, int D, int KMAX>
+void launch_knn(const T* pts, const int32_t* split, int d, const int32_t* qidx, const int32_t* midx, int64_t q0,
+                int64_t nq, int k, int32_t* out, hipStream_t s) {
+  hipLaunchKernelGGL((knn_kernel<T, D, KMAX>), dim3(grid_for(nq, 256)), dim3(256), 0, s, pts, split, d, qidx, midx,
+                     q0, nq, k, out);
+}
+
+template <typename T, int D>
+void dispatch_k(const T* pts, const int32_t* split, int d, const int32_t* qidx, const int32_t* midx, int64_t q0,
+                int64_t nq, int k, int32_t* out, hipStream_t s) {
+  if (k <= 8) launch_knn<T, D, 8>(pts, split, d, qidx, midx, q0, nq, k, out, s);
+  else if (k <= 16) launch_knn<T, D, 16>(pts, split, d, qidx, midx, q0, nq, k, out, s);
+  else if (k <= 32) launch_knn<T, D, 32>(pts, split, d, qidx, midx, q0, nq, k, out, s);
+  else launch_knn<T, D, 64>(pts, split, d, qidx, midx, q0, nq, k, out, s);
+}
+
+template <typename T>
+int knn_impl(const T* pts, const int32_t* split, int64_t n, int d, const int32_t* qidx, const int32_t* midx,
+             int64_t q0, int64_t nq, int k, int32_t* out, void* ws, size_t ws_bytes, hipStream_t s) {
+  GGP_CHECK_ARG(d >= 1 && d <= 8, "query_neighbors: 1 <= d <= 8");
+  GGP_CHECK_ARG(k >= 1, "query_neighbors: k >= 1");
+  GGP_CHECK_ARG(n < (int64_t(1) << 31), "query_neighbors: N must be below 2^31");
+  if (nq <= 0) return kOk;
+  if (k > 64) {
+    Workspace w(ws, ws_bytes);
+    T* wsd = w.take<T>((size_t)nq * k);
+    int32_t* wsi = w.take<int32_t>((size_t)nq * k);
+    if (!w.ok()) {
+      set_error("query_neighbors: workspace too small for k > 64");
+      return kWorkspaceTooSmall;
+    }
+    hipLaunchKernelGGL(knn_generic_kernel<T>, dim3(grid_for(nq, 256)), dim3(256), 0, s, pts, split, d, qidx, midx,
+                       q0, nq, k, out, wsd, wsi);
+  } else if (d == 1) {
+    dispatch_k<T, 1>(pts, split, d, qidx, midx, q0, nq, k, out, s);
+  } else if (d == 2) {
+    dispatch_k<T, 2>(pts, split, d, qidx, midx, q0, nq, k, out, s);
+  } else if (d == 3) {
+    dispatch_k<T, 3>(pts, split, d, qidx, midx, q0, nq, k, out, s);
+  } else {
+    dispatch_k<T, 8>(pts, split, d, qidx, midx, q0, nq, k, out, s);
+  }
+  GGP_LAUNCH_CHECK();
+  return kOk;
+}
+
+}  // namespace
+
+size_t query_neighbors_workspace(int64_t nq, int k, int dtype) {
+  if (k <= 64) return 0;
+  Workspace w(nullptr, 0);
+  w.take<char>((size_t)nq * k * dtype_size(dtype));
+  w.take<int32_t>((size_t)nq * k);
+  return w.used + 256;
+}
+
+int query_neighbors(const void* pts, const int32_t* split, int64_t n, int d, int dtype, const int32_t* qidx,
+                    const int32_t* midx, int64_t q0, int64_t nq, int k, int32_t* out, void* ws, size_t ws_bytes,
+                    hipStream_t s) {
+  if (dtype == 0)
+    return knn_impl<float>((const float*)pts, split, n, d, qidx, midx, q0, nq, k, out, ws, ws_bytes, s);
+  if (dtype == 1)
+    return knn_impl<double>((const double*)pts, split, n, d, qidx, midx, q0, nq, k, out, ws, ws_bytes, s);
+  set_error("query_neighbors: dtype must be 0 (float32) or 1 (float64)");
+  return kInvalidArgument;
+}
+
+}  // namespace ggp

@@ -1,0 +1,585 @@
+// refine.hip -- fused Vecchia refinement for gfx950: generate, inverse and log-det.
+//
+// Reference: graphgp/refine.py:79-159 (refine), :198-231 (refine_inv), :251-275
+// (refine_logdet), :288-305 (compute_cov_matrix / cov_lookup).  For refined point i
+// with neighbours n_1..n_k the joint covariance K over (n_1..n_k, i) (neighbours first,
+// fine point last) is built from the tabulated covariance by linear interpolation of
+// r = ||a - b||; with L = chol(K), mean = L[k,:k] . L_A^{-1} v[nbrs] and
+// std = L[k,k]; generate writes mean + std * xi, the inverse (v - mean) / std and the
+// log-det sums log std.
+//
+// MI355X mapping (one kernel per call, no per-batch launches):
+//  * P lanes cooperate on one point; row i of the (k+1)^2 system lives in lane i % P,
+//    slot i / P, holding columns 0..P*slot+P-1 (VGPRs, compile-time indexed).
+//  * Matrix entries are produced on the fly: the neighbour coordinates are gathered
+//    once (one row per lane) and broadcast across the P lanes with DPP quad_perm
+//    (P <= 4) or ds_swizzle (P = 8); the covariance table sits in LDS as
+//    {x_j, f_j, slope_j, x_j+1} segments and r is located by a log2 guess plus an
+//    exact fix-up (binary search when the bins are not log spaced).
+//  * Right-looking Cholesky on the row-cyclic layout; the forward substitution of the
+//    neighbour values runs on the same factor; the fine point's row gives mean/std.
+//  * generate: workgroups take tiles from an atomic ticket in index order; a refined
+//    value is published with an agent-scope store and a lane waits (polling, never
+//    blocking the rest of its wavefront) only for its own neighbours' values, which
+//    belong to lower tickets.  Any graph in topological order completes; one batch
+//    ordering (offsets) is not needed, and the result equals the reference's
+//    batch-serial sweep because every value is a deterministic function of earlier
+//    values.  A bounded spin turns a malformed graph into an error, never a hang.
+//  * Precision mode "increment" (FP32 only): the system is rewritten in increments
+//    relative to the nearest neighbour, B_ij = g(r_i0) + g(r_0j) - g(r_ij) with the
+//    variogram table g = f_0 - f, so that the entries keep their relative precision
+//    when k(r) ~ k(0) (clustered points); the value is v_0 + mean(increment) + std*xi.
+#include "common.hpp"
+#include "graphgp_amd_internal.hpp"
+#include "refine_impl.hpp"
+#include "dense.hpp"
+#include "refine_ops.hpp"
+
+namespace ggp {
+
+namespace {
+
+// ---------------------------------------------------------------- table prep
+template <typename T>
+__global__ void prep_table_kernel(const double* __restrict__ bins, const double* __restrict__ vals, int nb,
+                                  int increment, TabE<T>* __restrict__ tab) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nb - 1) return;
+  const double f0 = vals[0];
+  double x0 = bins[j], x1 = bins[j + 1];
+  double fa = vals[j], fb = vals[j + 1];
+  if (increment) {
+    fa = f0 - fa;
+    fb = f0 - fb;
+  }
+  double dx = x1 - x0;
+  double s = (fabs(dx) <= 1e-300) ? 0.0 : (fb - fa) / dx;
+  TabE<T> e;
+  e.x0 = (T)x0;
+  e.f0 = (T)fa;
+  e.s = (T)s;
+  e.x1 = (T)x1;
+  tab[j] = e;
+}
+
+__global__ void prep_meta_kernel(const double* __restrict__ bins, const double* __restrict__ vals, int nb,
+                                 int increment, TabMeta* __restrict__ meta) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  TabMeta m;
+  m.nseg = nb - 1;
+  m.x_first = bins[0];
+  m.x_last = bins[nb - 1];
+  m.f_first = increment ? 0.0 : vals[0];
+  m.f_last = increment ? vals[0] - vals[nb - 1] : vals[nb - 1];
+  m.f_zero = vals[0];
+  // log-spacing guess over bins[1..nb-1]
+  if (nb >= 3 && bins[1] > 0.0 && bins[nb - 1] > bins[1]) {
+    double l1 = log2(bins[1]), l2 = log2(bins[nb - 1]);
+    m.lb1 = (float)l1;
+    m.inv_step = (float)((double)(nb - 2) / (l2 - l1));
+    m.x1 = bins[1];
+  } else {
+    m.lb1 = 0.f;
+    m.inv_step = 0.f;
+    m.x1 = bins[nb > 1 ? 1 : 0];
+  }
+  *meta = m;
+}
+
+}  // namespace
+
+size_t table_workspace(int nb, int t64) { return (size_t)(nb) * (t64 ? sizeof(TabE<double>) : sizeof(TabE<float>)) + 512; }
+
+int prep_table(const double* bins, const double* vals, int nb, int t64, int increment, void* tab, TabMeta* meta,
+               hipStream_t s) {
+  GGP_CHECK_ARG(nb >= 2, "covariance table needs at least two bins");
+  if (t64)
+    hipLaunchKernelGGL(prep_table_kernel<double>, dim3(grid_for(nb, 256)), dim3(256), 0, s, bins, vals, nb,
+                       increment, (TabE<double>*)tab);
+  else
+    hipLaunchKernelGGL(prep_table_kernel<float>, dim3(grid_for(nb, 256)), dim3(256), 0, s, bins, vals, nb,
+                       increment, (TabE<float>*)tab);
+  hipLaunchKernelGGL(prep_meta_kernel, dim3(1), dim3(64), 0, s, bins, vals, nb, increment, meta);
+  GGP_LAUNCH_CHECK();
+  return kOk;
+}
+
+
+// ================================================================== generic k
+// Any k (and d up to 8): one lane per point, the (k+1)^2 system in a per-lane slab of
+// the workspace.  Used for k + 1 > 40 (e.g. k = n0 = 200 in the dense-limit check).
+namespace {
+
+template <typename T>
+__global__ __launch_bounds__(64) void refine_generic_kernel(RefineArgs<T> a, T* __restrict__ scratch) {
+  const int k = a.k, m = k + 1;
+  T* K = scratch + (size_t)(blockIdx.x * 64 + threadIdx.x) * ((size_t)m * m + m);
+  T* u = K + (size_t)m * m;
+  const TabMeta meta = *a.meta;
+  const T x1 = (T)meta.x1, x_last = (T)meta.x_last, f_last = (T)meta.f_last;
+  __shared__ uint32_t s_tile;
+  const bool incr = a.increment != 0;
+  for (;;) {
+    if (threadIdx.x == 0) s_tile = atomicAdd(a.ticket, 1u);
+    __syncthreads();
+    const int64_t tile = s_tile;
+    __syncthreads();
+    if (tile >= a.ntiles) break;
+    const int64_t pt = tile * 64 + threadIdx.x;
+    const bool valid = pt < a.n_ref;
+    const int64_t fine = a.n0 + pt;
+    auto pid = [&](int i) -> int64_t { return i < k ? (int64_t)a.nbrs[pt * k + i] : fine; };
+    auto coord = [&](int64_t id, int t) { return load_as<T>(a.pts, a.pts_f64, id * a.d + t); };
+    auto dist = [&](int i, int j) {
+      int64_t p = pid(i), q = pid(j);
+      T s = T(0);
+      for (int t = 0; t < a.d; ++t) {
+        T dd = coord(p, t) - coord(q, t);
+        s += dd * dd;
+      }
+      return sqrt(s);
+    };
+    auto look = [&](T r) { return table_lookup<T>(a.tab, meta.nseg, x1, x_last, f_last, meta.lb1, meta.inv_step, r); };
+    T skk = T(1);
+    if (valid) {
+      for (int i = 0; i < m; ++i) {
+        T gi = incr ? look(dist(i, 0)) : T(0);
+        for (int j = 0; j <= i; ++j) {
+          T v = look(dist(i, j));
+          if (incr) {
+            T gj = look(dist(j, 0));
+            if (j == 0) v = (i == 0) ? (T)meta.f_zero : -gi;
+            else v = (gi + gj) - v;
+          }
+          K[(size_t)i * m + j] = v;
+        }
+      }
+      for (int c = 0; c < k; ++c) {
+        T dc = sqrt(K[(size_t)c * m + c]);
+        K[(size_t)c * m + c] = dc;
+        for (int i = c + 1; i < m; ++i) K[(size_t)i * m + c] /= dc;
+        for (int i = c + 1; i < m; ++i) {
+          T lic = K[(size_t)i * m + c];
+          for (int j = c + 1; j <= i; ++j) K[(size_t)i * m + j] -= lic * K[(size_t)j * m + c];
+        }
+      }
+      skk = K[(size_t)k * m + k];
+    }
+    auto forward = [&]() -> T {
+      for (int c = 0; c < k; ++c) {
+        T y = u[c] / K[(size_t)c * m + c];
+        for (int i = c + 1; i < m; ++i) u[i] -= K[(size_t)i * m + c] * y;
+      }
+      return u[k];
+    };
+    if (a.mode == kLogdet) {
+      double ld = valid ? 0.5 * log((double)skk) : 0.0;
+      if (valid && ld != ld) atomicOr(&a.flags[0], 1u);
+      for (int o = 32; o > 0; o >>= 1) ld += __shfl_xor(ld, o, 64);
+      if (threadIdx.x == 0) a.partials[tile] = ld;
+      continue;
+    }
+    const int64_t dst = valid ? (a.perm ? (int64_t)a.perm[fine] : fine) : 0;
+    if (a.mode == kInverse) {
+      if (valid) {
+        for (int i = 0; i < k; ++i) u[i] = a.vals[pid(i)];
+        u[k] = a.vals[fine];
+        if (incr)
+          for (int i = 1; i <= k; ++i) u[i] -= u[0];
+        T xi = forward() / sqrt(skk);
+        if (xi != xi) atomicOr(&a.flags[0], 1u);
+        if (a.out_f64) ((double*)a.out)[dst - a.io_base] = (double)xi;
+        else ((float*)a.out)[dst - a.io_base] = (float)xi;
+      }
+      continue;
+    }
+    const T xi = valid ? load_as<T>(a.xi, a.xi_f64, dst - a.io_base) : T(0);
+    bool done = !valid;
+    for (uint32_t it = 0;; ++it) {
+      if (!done) {
+        bool ok = true;
+        for (int i = 0; i < k; ++i) {
+          T v = __hip_atomic_load(&a.vals[pid(i)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ok = ok && !Sentinel<T>::is(v);
+          u[i] = v;
+        }
+        if (ok) {
+          u[k] = T(0);
+          T v0 = T(0);
+          if (incr) {
+            v0 = u[0];
+            for (int i = 1; i < k; ++i) u[i] -= v0;
+          }
+          T v = v0 - forward() + sqrt(skk) * xi;
+          if (v != v) {
+            atomicOr(&a.flags[0], 1u);
+            v = __builtin_nan("");
+          }
+          __hip_atomic_store(&a.vals[fine], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (a.out) {
+            if (a.out_f64) ((double*)a.out)[dst] = (double)v;
+            else ((float*)a.out)[dst] = (float)v;
+          }
+          done = true;
+        }
+      }
+      if (__all(done)) break;
+      if (it > (1u << 24)) {
+        if (!done) {
+          atomicOr(&a.flags[1], 1u);
+          __hip_atomic_store(&a.vals[fine], (T)__builtin_nan(""), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+  }
+}
+
+constexpr int kGenericBlocks = 256;
+
+template <typename T>
+size_t generic_scratch_bytes(int k) {
+  const size_t m = (size_t)k + 1;
+  return (size_t)kGenericBlocks * 64 * (m * m + m) * sizeof(T);
+}
+
+// ------------------------------------------------------------------ small kernels
+template <typename T>
+__global__ void seed_fill_kernel(T* __restrict__ vals, const void* __restrict__ init, int init_f64, int64_t n0,
+                                 int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  vals[i] = i < n0 ? load_as<T>(init, init_f64, i) : Sentinel<T>::value();
+}
+
+template <typename T>
+__global__ void gather_vals_kernel(const void* __restrict__ src, int src_f64, const int32_t* __restrict__ perm,
+                                   int64_t n, T* __restrict__ dst) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  dst[i] = load_as<T>(src, src_f64, perm ? (int64_t)perm[i] : i);
+}
+
+__global__ void gather_seed_xi_kernel(const void* __restrict__ xi, int xi_f64, const int32_t* __restrict__ perm,
+                                      int64_t n0, double* __restrict__ xs) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n0) return;
+  xs[i] = load_as<double>(xi, xi_f64, perm ? (int64_t)perm[i] : i);
+}
+
+template <typename T>
+__global__ void seed_out_kernel(const double* __restrict__ sv, const int32_t* __restrict__ perm, int64_t n0,
+                                T* __restrict__ vals, void* __restrict__ out, int out_f64, uint32_t* flags) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n0) return;
+  const double v = sv[i];
+  if (v != v) atomicOr(&flags[0], 1u);
+  if (vals) vals[i] = (T)v;
+  if (out) {
+    const int64_t dst = perm ? (int64_t)perm[i] : i;
+    if (out_f64) ((double*)out)[dst] = v;
+    else ((float*)out)[dst] = (float)v;
+  }
+}
+
+template <typename T>
+__global__ void seed_vals_to_f64_kernel(const T* __restrict__ vals, int64_t n0, double* __restrict__ v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n0) v[i] = (double)vals[i];
+}
+
+__global__ void nan_fill_kernel(void* __restrict__ out, int out_f64, int64_t n, const uint32_t* __restrict__ flags) {
+  if (flags[0] == 0) return;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    if (out_f64) ((double*)out)[i] = __builtin_nan("");
+    else ((float*)out)[i] = __builtin_nanf("");
+  }
+}
+
+__global__ void logdet_sum_kernel(const double* __restrict__ partials, int64_t np, const double* __restrict__ seed,
+                                  double* __restrict__ out) {
+  __shared__ double red[16];
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < np; i += 1024) s += partials[i];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < 16; ++w) t += red[w];
+    *out = seed ? t + *seed : t;
+  }
+}
+
+int device_cus() {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      cus = 256;
+  }
+  return cus;
+}
+
+size_t max_lds() {
+  static size_t lds = 0;
+  if (lds == 0) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+      lds = (size_t)v;
+    else
+      lds = 65536;
+  }
+  return lds;
+}
+
+// workspace layout shared by all refine flows
+template <typename T>
+struct RefineWs {
+  TabE<T>* tab = nullptr;
+  TabMeta* meta = nullptr;
+  uint32_t* ctr = nullptr;  // [0] ticket, [1] NaN flag, [2] error flag
+  double* partials = nullptr;
+  T* vals = nullptr;  // graph-order values (N)
+  double* dense = nullptr;
+  double* dvec = nullptr;  // n0
+  double* dout = nullptr;  // n0 (+2 scalars)
+  T* scratch = nullptr;
+};
+
+template <typename T>
+bool carve(Workspace& w, const RefineProblem& p, int mode, RefineWs<T>& b) {
+  b.tab = w.take<TabE<T>>(p.nb);
+  b.meta = w.take<TabMeta>(1);
+  b.ctr = w.take<uint32_t>(4);
+  const int64_t n0 = p.n - p.n_ref;
+  b.partials = w.take<double>((p.n_ref + 31) / 32 + 1);
+  if (mode != kLogdet || p.with_seed) b.vals = w.take<T>(p.n);
+  if (p.with_seed) {
+    b.dense = w.take<double>((size_t)n0 * n0);
+    b.dvec = w.take<double>(n0 + 1);
+    b.dout = w.take<double>(n0 + 4);
+  }
+  if (p.k + 1 > 40 || p.d > 8 || (p.d > 3 && p.k + 1 > 16)) b.scratch = w.take<T>(generic_scratch_bytes<T>(p.k) / sizeof(T));
+  return w.ok();
+}
+
+template <typename T>
+int run_kernel(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hipStream_t s) {
+  a.pts = p.pts;
+  a.pts_f64 = p.pts_f64;
+  a.d = p.d;
+  a.nbrs = p.nbrs;
+  a.k = p.k;
+  a.n0 = p.n - p.n_ref;
+  a.n_ref = p.n_ref;
+  a.tab = b.tab;
+  a.meta = b.meta;
+  a.ticket = b.ctr;
+  a.flags = b.ctr + 1;
+  a.partials = b.partials;
+  a.increment = p.increment;
+  if (p.n_ref == 0) return kOk;
+  const int kk = p.k + 1;
+  const bool generic = kk > 40 || p.d > 8 || (p.d > 3 && kk > 16);
+  if (generic) {
+    a.ntiles = (p.n_ref + 63) / 64;
+    const int grid = (int)std::min<int64_t>(a.ntiles, kGenericBlocks);
+    hipLaunchKernelGGL(refine_generic_kernel<T>, dim3(grid), dim3(64), 0, s, a, b.scratch);
+    GGP_LAUNCH_CHECK();
+    return kOk;
+  }
+  const size_t lds = (size_t)(p.nb - 1) * sizeof(TabE<T>);
+  if (lds > max_lds()) {
+    set_error("refine: covariance table does not fit in LDS (" + std::to_string(p.nb) + " bins)");
+    return kUnsupported;
+  }
+  const int grid = device_cus() * 4;
+  int rc = launch_refine<T>(a, kk, s, lds, grid);
+  if (rc == kUnsupported && ggp::last_error()[0] == 0) set_error("refine: unsupported configuration");
+  return rc;
+}
+
+template <typename T>
+int refine_flow(const RefineProblem& p, int mode, const RefineIO& io, void* ws, size_t ws_bytes, hipStream_t s) {
+  GGP_CHECK_ARG(p.n >= 1 && p.n_ref >= 0 && p.n_ref <= p.n && p.k >= 1 && p.d >= 1, "refine: bad sizes");
+  GGP_CHECK_ARG(p.nb >= 2, "refine: covariance table needs at least two bins");
+  GGP_CHECK_ARG(p.n < (int64_t(1) << 31), "refine: N must be below 2^31");
+  GGP_CHECK_ARG(!(p.increment && sizeof(T) == 8), "refine: the increment formulation is an FP32 mode");
+  Workspace w(ws, ws_bytes);
+  RefineWs<T> b;
+  if (!carve<T>(w, p, mode, b)) {
+    set_error("refine: workspace too small");
+    return kWorkspaceTooSmall;
+  }
+  const int64_t n0 = p.n - p.n_ref;
+  GGP_TRY(fill_u32(b.ctr, 0u, 4, s));
+  GGP_TRY(prep_table(p.bins, p.vals, p.nb, sizeof(T) == 8, p.increment, b.tab, b.meta, s));
+  RefineArgs<T> a{};
+  a.mode = mode;
+  a.perm = p.perm;
+  if (mode == kGenerate) {
+    // graph-order buffer: seeds (given or from the dense block) then sentinels
+    if (p.with_seed) {
+      hipLaunchKernelGGL(seed_fill_kernel<T>, dim3(grid_for(p.n, 256)), dim3(256), 0, s, b.vals, (const void*)nullptr,
+                         0, (int64_t)0, p.n);
+      GGP_LAUNCH_CHECK();
+      GGP_TRY(dense_factor(p.pts, p.pts_f64, p.d, n0, p.bins, p.vals, p.nb, b.dense, s));
+      hipLaunchKernelGGL(gather_seed_xi_kernel, dim3(grid_for(n0, 256)), dim3(256), 0, s, io.xi, io.xi_f64, p.perm,
+                         n0, b.dvec);
+      GGP_LAUNCH_CHECK();
+      GGP_TRY(dense_apply(b.dense, n0, b.dvec, b.dout, s));
+      hipLaunchKernelGGL(seed_out_kernel<T>, dim3(grid_for(n0, 256)), dim3(256), 0, s, b.dout, p.perm, n0, b.vals,
+                         io.out, io.out_f64, b.ctr + 1);
+      GGP_LAUNCH_CHECK();
+      a.vals = b.vals;
+      a.out = io.out;
+      a.io_base = 0;
+    } else {
+      T* vals = static_cast<T*>(io.out);  // raw refine: output is the graph-order values array
+      hipLaunchKernelGGL(seed_fill_kernel<T>, dim3(grid_for(p.n, 256)), dim3(256), 0, s, vals, io.init, io.init_f64,
+                         n0, p.n);
+      GGP_LAUNCH_CHECK();
+      a.vals = vals;
+      a.out = nullptr;
+      a.io_base = n0;
+    }
+    a.xi = io.xi;
+    a.xi_f64 = io.xi_f64;
+    a.out_f64 = io.out_f64;
+    GGP_TRY(run_kernel<T>(a, p, b, s));
+    if (p.with_seed) {
+      hipLaunchKernelGGL(nan_fill_kernel, dim3(grid_for(std::min<int64_t>(p.n, 1 << 20), 256)), dim3(256), 0, s,
+                         io.out, io.out_f64, p.n, b.ctr + 1);
+      GGP_LAUNCH_CHECK();
+    }
+    return kOk;
+  }
+  if (mode == kInverse) {
+    hipLaunchKernelGGL(gather_vals_kernel<T>, dim3(grid_for(p.n, 256)), dim3(256), 0, s, io.values, io.values_f64,
+                       p.perm, p.n, b.vals);
+    GGP_LAUNCH_CHECK();
+    a.vals = b.vals;
+    a.out = io.out;
+    a.out_f64 = io.out_f64;
+    a.io_base = p.with_seed ? 0 : n0;
+    GGP_TRY(run_kernel<T>(a, p, b, s));
+    if (p.with_seed) {
+      GGP_TRY(dense_factor(p.pts, p.pts_f64, p.d, n0, p.bins, p.vals, p.nb, b.dense, s));
+      hipLaunchKernelGGL(seed_vals_to_f64_kernel<T>, dim3(grid_for(n0, 256)), dim3(256), 0, s, b.vals, n0, b.dvec);
+      GGP_LAUNCH_CHECK();
+      GGP_TRY(dense_solve(b.dense, n0, b.dvec, s));
+      hipLaunchKernelGGL(seed_out_kernel<T>, dim3(grid_for(n0, 256)), dim3(256), 0, s, b.dvec, p.perm, n0,
+                         (T*)nullptr, io.out, io.out_f64, b.ctr + 1);
+      GGP_LAUNCH_CHECK();
+      hipLaunchKernelGGL(nan_fill_kernel, dim3(grid_for(std::min<int64_t>(p.n, 1 << 20), 256)), dim3(256), 0, s,
+                         io.out, io.out_f64, p.n, b.ctr + 1);
+      GGP_LAUNCH_CHECK();
+    } else if (io.init) {
+      // raw refine_inv: the initial values are the first n0 values
+      hipLaunchKernelGGL(seed_vals_to_f64_kernel<T>, dim3(grid_for(n0, 256)), dim3(256), 0, s, b.vals, n0,
+                         (double*)io.init);
+      GGP_LAUNCH_CHECK();
+    }
+    return kOk;
+  }
+  // log-det
+  a.vals = nullptr;
+  GGP_TRY(run_kernel<T>(a, p, b, s));
+  const int64_t np = p.n_ref == 0 ? 0 : (p.k + 1 > 40 || p.d > 8 || (p.d > 3 && p.k + 1 > 16)) ? (p.n_ref + 63) / 64 : -1;
+  int64_t ntiles = np;
+  if (ntiles < 0) {
+    // tiles of the register kernel: points per tile = 256 / P for the chosen configuration
+    ntiles = (p.n_ref + refine_points_per_tile(sizeof(T) == 8, p.k + 1, p.d) - 1) /
+             refine_points_per_tile(sizeof(T) == 8, p.k + 1, p.d);
+  }
+  const double* seed = nullptr;
+  if (p.with_seed && n0 > 0) {
+    GGP_TRY(dense_factor(p.pts, p.pts_f64, p.d, n0, p.bins, p.vals, p.nb, b.dense, s));
+    GGP_TRY(dense_logdiag(b.dense, n0, b.dout, s));
+    seed = b.dout;
+  }
+  hipLaunchKernelGGL(logdet_sum_kernel, dim3(1), dim3(1024), 0, s, b.partials, ntiles, seed, (double*)io.out);
+  GGP_LAUNCH_CHECK();
+  return kOk;
+}
+
+}  // namespace
+
+int refine_points_per_tile(int t64, int kk, int d) {
+  if (kk > 40 || d > 8 || (d > 3 && kk > 16)) return 64;
+  if (!t64) {
+    if (d > 3) return kk <= 8 ? 256 : 128;
+    if (kk <= 12) return 256;
+    if (kk <= 20) return 128;
+    if (kk <= 32) return 64;
+    return 32;
+  }
+  if (d > 3) return kk <= 8 ? 128 : 64;
+  if (kk <= 4) return 256;
+  if (kk <= 12) return 128;
+  if (kk <= 24) return 64;
+  return 32;
+}
+
+size_t refine_workspace(const RefineProblem& p, int mode, int t64) {
+  Workspace w(nullptr, 0);
+  if (t64) {
+    RefineWs<double> b;
+    carve<double>(w, p, mode, b);
+  } else {
+    RefineWs<float> b;
+    carve<float>(w, p, mode, b);
+  }
+  return w.used + 256;
+}
+
+int refine_run(const RefineProblem& p, int mode, int t64, const RefineIO& io, void* ws, size_t ws_bytes,
+               hipStream_t s) {
+  return t64 ? refine_flow<double>(p, mode, io, ws, ws_bytes, s) : refine_flow<float>(p, mode, io, ws, ws_bytes, s);
+}
+
+// Dense-only flows (generate_dense / generate_dense_inv / generate_dense_logdet).
+size_t dense_flow_workspace(int64_t n0) {
+  Workspace w(nullptr, 0);
+  w.take<double>((size_t)n0 * n0);
+  w.take<double>(n0 + 1);
+  w.take<double>(n0 + 4);
+  w.take<uint32_t>(4);
+  return w.used + 256;
+}
+
+int dense_flow(const void* pts, int pts_f64, int d, int64_t n0, const double* bins, const double* vals, int nb,
+               int mode, const void* in, int in_f64, void* out, int out_f64, void* ws, size_t ws_bytes,
+               hipStream_t s) {
+  GGP_CHECK_ARG(n0 >= 1 && d >= 1 && nb >= 2, "dense: bad sizes");
+  Workspace w(ws, ws_bytes);
+  double* A = w.take<double>((size_t)n0 * n0);
+  double* v = w.take<double>(n0 + 1);
+  double* o = w.take<double>(n0 + 4);
+  uint32_t* flags = w.take<uint32_t>(4);
+  if (!w.ok()) {
+    set_error("dense: workspace too small");
+    return kWorkspaceTooSmall;
+  }
+  GGP_TRY(fill_u32(flags, 0u, 4, s));
+  GGP_TRY(dense_factor(pts, pts_f64, d, n0, bins, vals, nb, A, s));
+  if (mode == kLogdet) return dense_logdiag(A, n0, (double*)out, s);
+  hipLaunchKernelGGL(gather_seed_xi_kernel, dim3(grid_for(n0, 256)), dim3(256), 0, s, in, in_f64,
+                     (const int32_t*)nullptr, n0, v);
+  GGP_LAUNCH_CHECK();
+  if (mode == kGenerate) {
+    GGP_TRY(dense_apply(A, n0, v, o, s));
+  } else {
+    GGP_TRY(dense_solve(A, n0, v, s));
+    o = v;
+  }
+  hipLaunchKernelGGL(seed_out_kernel<double>, dim3(grid_for(n0, 256)), dim3(256), 0, s, o, (const int32_t*)nullptr,
+                     n0, (double*)nullptr, out, out_f64, flags);
+  GGP_LAUNCH_CHECK();
+  return kOk;
+}
+
+}  // namespace ggp

@@ -1,0 +1,27 @@
+// refine_launch.hpp -- instantiation helper for the refine kernel configurations.
+#pragma once
+#include "refine_impl.hpp"
+
+namespace ggp {
+
+template <typename T, int KK, int P, int D>
+int launch_config(RefineArgs<T> a, hipStream_t s, size_t lds_bytes, int grid) {
+  auto kern = refine_kernel<T, KK, P, D>;
+  constexpr int64_t gpt = 256 / P;
+  a.ntiles = (a.n_ref + gpt - 1) / gpt;
+  if (a.ntiles == 0) return kOk;
+  if ((int64_t)grid > a.ntiles) grid = (int)a.ntiles;
+  if (lds_bytes > 65536) {
+    hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    if (e != hipSuccess) {
+      set_error(std::string("refine: cannot reserve LDS for the covariance table: ") + hipGetErrorString(e));
+      return kUnsupported;
+    }
+  }
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds_bytes, s, a);
+  GGP_LAUNCH_CHECK();
+  return kOk;
+}
+
+
+}  // namespace ggp

@@ -1,0 +1,43 @@
+// refine_ops.hpp -- host-side refinement flows (refine.hip).
+#pragma once
+#include "common.hpp"
+
+namespace ggp {
+
+struct RefineProblem {
+  const void* pts;  // graph-order points (n, d)
+  int pts_f64;
+  int64_t n;
+  int d;
+  const int32_t* nbrs;  // (n_ref, k)
+  int64_t n_ref;
+  int k;
+  const int32_t* perm;  // graph.indices or null
+  const double* bins;
+  const double* vals;
+  int nb;
+  int increment;  // FP32 increment formulation
+  int with_seed;  // 1: generate/generate_inv/generate_logdet (dense seed block); 0: raw refine*
+};
+
+struct RefineIO {
+  const void* xi;  // generate: xi (caller order, N) or raw refine: xi (n_ref)
+  int xi_f64;
+  const void* values;  // inverse: values (caller order)
+  int values_f64;
+  void* init;  // raw refine: initial values in (n0); raw refine_inv: initial values out (n0, FP64)
+  int init_f64;
+  void* out;  // generate: values; inverse: xi; log-det: one FP64 scalar
+  int out_f64;
+};
+
+int refine_points_per_tile(int t64, int kk, int d);
+size_t refine_workspace(const RefineProblem& p, int mode, int t64);
+int refine_run(const RefineProblem& p, int mode, int t64, const RefineIO& io, void* ws, size_t ws_bytes,
+               hipStream_t s);
+size_t dense_flow_workspace(int64_t n0);
+int dense_flow(const void* pts, int pts_f64, int d, int64_t n0, const double* bins, const double* vals, int nb,
+               int mode, const void* in, int in_f64, void* out, int out_f64, void* ws, size_t ws_bytes,
+               hipStream_t s);
+
+}  // namespace ggp

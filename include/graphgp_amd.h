@@ -1,0 +1,160 @@
+/* graphgp_amd.h -- C ABI of the MI355X (gfx950) GraphGP hot path.
+ *
+ * One entry point per kernel family behind the reference's Python API
+ * (stanford-ism/graphgp).  The reference crosses into its optional native extension
+ * (`graphgp_cuda`) at nine call sites; each entry below names the call site it replaces.
+ *
+ * Conventions
+ *  - All array arguments are DEVICE pointers; sizes are int64_t; `stream` is a
+ *    hipStream_t (NULL = default stream).  Work is enqueued on `stream` only; no entry
+ *    point synchronises except where a host result is documented (marked "syncs").
+ *  - No entry point allocates device memory.  Scratch comes from a caller-provided
+ *    `workspace` of at least `*_workspace(...)` bytes (256-byte aligned is enough).
+ *  - dtype codes: GGP_F32 = 0 (float32), GGP_F64 = 1 (float64).
+ *  - Return value: 0 on success, negative on error; ggp_last_error() returns the
+ *    message (per host thread).  Numerical failure (a non-positive-definite local
+ *    covariance) is not an error: as in the reference (refine.py:55, :182) the whole
+ *    output of generate / generate_inv becomes NaN.
+ *  - Index arrays are int32 (N < 2^31).
+ */
+#ifndef GRAPHGP_AMD_H
+#define GRAPHGP_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GGP_F32 0
+#define GGP_F64 1
+
+/* Precision of the refinement arithmetic. */
+#define GGP_PREC_FP64 0      /* FP64 local systems (the reference formulation)               */
+#define GGP_PREC_FP32 1      /* FP32 local systems (the reference formulation)               */
+#define GGP_PREC_FP32_INC 2  /* FP32 in increments relative to the nearest neighbour         */
+
+const char* ggp_last_error(void);
+int ggp_abi_version(void);
+
+/* ---- k-d tree: replaces graphgp_cuda.build_tree(points), graphgp/tree.py:37 ------------
+ * points (n, d) -> points_out (n, d) in tree order, split_dims (n), indices (n). */
+size_t ggp_build_tree_workspace(int64_t n, int d, int dtype);
+int ggp_build_tree(const void* points, int64_t n, int d, int dtype, void* points_out, int32_t* split_dims,
+                   int32_t* indices, void* workspace, size_t workspace_bytes, void* stream);
+
+/* ---- preceding kNN: replaces graphgp_cuda.query_preceding_neighbors(points, split_dims,
+ * n0=, k=), graphgp/tree.py:61.  neighbors (n - n0, k). */
+size_t ggp_query_neighbors_workspace(int64_t n_queries, int k, int dtype);
+int ggp_query_preceding_neighbors(const void* points, const int32_t* split_dims, int64_t n, int d, int dtype,
+                                  int64_t n0, int k, int32_t* neighbors, void* workspace, size_t workspace_bytes,
+                                  void* stream);
+
+/* ---- kNN with explicit query / max indices: replaces graphgp_cuda.query_neighbors(points,
+ * split_dims, query_indices, max_indices, k=), graphgp/tree.py:73.  neighbors (nq, k). */
+int ggp_query_neighbors(const void* points, const int32_t* split_dims, int64_t n, int d, int dtype,
+                        const int32_t* query_indices, const int32_t* max_indices, int64_t n_queries, int k,
+                        int32_t* neighbors, void* workspace, size_t workspace_bytes, void* stream);
+
+/* ---- depths: replaces graphgp_cuda.compute_depths_parallel(neighbors, n0=),
+ * graphgp/graph.py:106.  depths (n0 + n_ref).  A malformed graph (a neighbour that does
+ * not precede its point) is reported by ggp_compute_depths_status (syncs): bit 0 =
+ * non-preceding neighbour, bit 1 = wait limit reached. */
+size_t ggp_compute_depths_workspace(void);
+int ggp_compute_depths(const int32_t* neighbors, int64_t n_ref, int k, int64_t n0, int32_t* depths, void* workspace,
+                       size_t workspace_bytes, void* stream);
+int ggp_compute_depths_status(void* workspace, uint32_t* host_status, void* stream);
+
+/* ---- depth order: replaces graphgp_cuda.order_by_depth(points, indices, neighbors,
+ * depths), graphgp/graph.py:128.  Stable by depth; indices may be NULL (identity). */
+size_t ggp_order_by_depth_workspace(int64_t n);
+int ggp_order_by_depth(const void* points, int64_t n, int d, int dtype, const int32_t* indices,
+                       const int32_t* neighbors, int64_t n_ref, int k, const int32_t* depths, int32_t max_depth,
+                       void* points_out, int32_t* indices_out, int32_t* neighbors_out, int32_t* depths_out,
+                       void* workspace, size_t workspace_bytes, void* stream);
+
+/* ---- batch offsets of graphgp/graph.py:95-96: offsets[b] = #(depth <= b), b < n_offsets. */
+int ggp_depth_offsets(const int32_t* depths_sorted, int64_t n, int64_t n_offsets, int64_t* offsets, void* stream);
+
+/* ---- whole graph build: replaces graphgp_cuda.build_graph(points, n0=, k=),
+ * graphgp/graph.py:89 (tree, preceding kNN, depths, depth order).  Outputs are in graph
+ * (depth) order; *max_depth_host receives the largest depth (syncs once), from which
+ * the caller sizes offsets (max_depth + 1 entries) for ggp_depth_offsets. */
+size_t ggp_build_graph_workspace(int64_t n, int d, int dtype, int64_t n0, int k);
+int ggp_build_graph(const void* points, int64_t n, int d, int dtype, int64_t n0, int k, void* points_out,
+                    int32_t* indices_out, int32_t* neighbors_out, int32_t* depths_out, int32_t* max_depth_host,
+                    void* workspace, size_t workspace_bytes, void* stream);
+
+/* ---- graph validation of graphgp/graph.py:44-70 (check_graph).  *status_device receives
+ * bit 0: malformed offsets, bit 1: a neighbour does not precede its point, bit 2: a
+ * neighbour lies in its point's own batch. */
+int ggp_check_graph(const int32_t* neighbors, int64_t n_ref, int k, int64_t n, const int64_t* offsets,
+                    int64_t n_offsets, uint32_t* status_device, void* stream);
+
+/* ---- refinement ------------------------------------------------------------------------
+ * Shared arguments: points (n, d) in graph order (dtype pdtype), neighbors (n - n0, k),
+ * cov_bins / cov_vals (n_bins) FP64, precision = GGP_PREC_*.  `indices` is the graph's
+ * permutation (graph order -> caller order) or NULL.  Values / xi in caller order have
+ * the dtype given by their own *_dtype argument.  The refined points' values come from a
+ * single dataflow launch; `offsets` of the reference are therefore not an input. */
+size_t ggp_refine_workspace(int64_t n, int d, int64_t n0, int k, int n_bins, int precision, int with_seed);
+
+/* generate: replaces graphgp.generate (refine.py:21-56) including the dense seed block
+ * (refine.py:59-76) and graphgp_cuda.refine (refine.py:119).  xi (n) -> values (n). */
+int ggp_generate(const void* points, int pdtype, int64_t n, int d, const int32_t* neighbors, int64_t n0, int k,
+                 const int32_t* indices, const double* cov_bins, const double* cov_vals, int n_bins,
+                 const void* xi, int xi_dtype, void* values, int values_dtype, int precision, void* workspace,
+                 size_t workspace_bytes, void* stream);
+
+/* generate_inv: graphgp.generate_inv (refine.py:162-183), incl. graphgp_cuda.refine_inv
+ * (refine.py:216) and generate_dense_inv (refine.py:186-195).  values (n) -> xi (n). */
+int ggp_generate_inv(const void* points, int pdtype, int64_t n, int d, const int32_t* neighbors, int64_t n0, int k,
+                     const int32_t* indices, const double* cov_bins, const double* cov_vals, int n_bins,
+                     const void* values, int values_dtype, void* xi, int xi_dtype, int precision, void* workspace,
+                     size_t workspace_bytes, void* stream);
+
+/* generate_logdet: graphgp.generate_logdet (refine.py:234-240), incl.
+ * graphgp_cuda.refine_logdet (refine.py:265).  One FP64 scalar at logdet_device. */
+int ggp_generate_logdet(const void* points, int pdtype, int64_t n, int d, const int32_t* neighbors, int64_t n0, int k,
+                        const double* cov_bins, const double* cov_vals, int n_bins, double* logdet_device,
+                        int precision, void* workspace, size_t workspace_bytes, void* stream);
+
+/* refine: graphgp_cuda.refine(points, neighbors, offsets, cov_bins, cov_vals, initial_values,
+ * xi), refine.py:119.  initial_values (n0), xi (n - n0) -> values (n, graph order) with the
+ * compute dtype (FP64 for GGP_PREC_FP64, FP32 otherwise). */
+int ggp_refine(const void* points, int pdtype, int64_t n, int d, const int32_t* neighbors, int64_t n0, int k,
+               const double* cov_bins, const double* cov_vals, int n_bins, const void* initial_values,
+               int init_dtype, const void* xi, int xi_dtype, void* values, int precision, void* workspace,
+               size_t workspace_bytes, void* stream);
+
+/* refine_inv: graphgp_cuda.refine_inv(...), refine.py:216.  values (n, graph order) ->
+ * initial_values (n0, FP64) and xi (n - n0, dtype xi_dtype). */
+int ggp_refine_inv(const void* points, int pdtype, int64_t n, int d, const int32_t* neighbors, int64_t n0, int k,
+                   const double* cov_bins, const double* cov_vals, int n_bins, const void* values, int values_dtype,
+                   double* initial_values, void* xi, int xi_dtype, int precision, void* workspace,
+                   size_t workspace_bytes, void* stream);
+
+/* refine_logdet: graphgp_cuda.refine_logdet(...), refine.py:265.  One FP64 scalar. */
+int ggp_refine_logdet(const void* points, int pdtype, int64_t n, int d, const int32_t* neighbors, int64_t n0, int k,
+                      const double* cov_bins, const double* cov_vals, int n_bins, double* logdet_device,
+                      int precision, void* workspace, size_t workspace_bytes, void* stream);
+
+/* ---- dense seed block (always FP64): generate_dense / generate_dense_inv /
+ * generate_dense_logdet, refine.py:59-76, 186-195, 243-248. */
+size_t ggp_dense_workspace(int64_t n);
+int ggp_generate_dense(const void* points, int pdtype, int64_t n, int d, const double* cov_bins,
+                       const double* cov_vals, int n_bins, const void* xi, int xi_dtype, void* values,
+                       int values_dtype, void* workspace, size_t workspace_bytes, void* stream);
+int ggp_generate_dense_inv(const void* points, int pdtype, int64_t n, int d, const double* cov_bins,
+                           const double* cov_vals, int n_bins, const void* values, int values_dtype, void* xi,
+                           int xi_dtype, void* workspace, size_t workspace_bytes, void* stream);
+int ggp_generate_dense_logdet(const void* points, int pdtype, int64_t n, int d, const double* cov_bins,
+                              const double* cov_vals, int n_bins, double* logdet_device, void* workspace,
+                              size_t workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GRAPHGP_AMD_H */
