@@ -1,0 +1,241 @@
+"""Headline benchmark: refined points/s of ``generate`` with the graph resident in HBM.
+
+Workload (the large-dynamic-range precision run): N = 16,777,216 points in 3-D drawn from a
+mixture of 64 isotropic Gaussians (centres U[0,1]^3, widths log-uniform in [1e-4, 1e-1]),
+n0 = 1000, k = 30, Matern-3/2 covariance (variance 1, scale 0.05) tabulated at r = 0 plus
+2000 log bins over [1e-6, 2] with jitter 1e-5, xi ~ N(0,1) float32.  Synthetic data with
+fixed seeds.
+
+A step is one full ``generate`` call (FP64 seed block, fused refinement, scatter to the
+caller's order) on the resident graph.  ``value`` is FP32-increment throughput (refined
+points / s); the same line reports the plain FP64 and plain FP32 paths, their ratio, the
+error of the FP32 paths against the FP64 path (itself checked point by point against the
+FP64 oracle on a random sample), the refinement kernel's roofline figures from HIP
+events on its own stream, and the CPU baseline (the FP64 oracle restatement of the
+reference's fast_jit refinement on the host cores, bounded sample).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def clustered(n, d, n_clusters, wmin, wmax, seed):
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(0.0, 1.0, (n_clusters, d))
+    widths = np.exp(rng.uniform(np.log(wmin), np.log(wmax), n_clusters))
+    lab = rng.integers(0, n_clusters, n)
+    pts = np.empty((n, d), np.float32)
+    step = 1 << 22
+    for s in range(0, n, step):
+        e = min(n, s + step)
+        pts[s:e] = (centres[lab[s:e]] + widths[lab[s:e], None] * rng.standard_normal((e - s, d))).astype(np.float32)
+    return pts
+
+
+def algorithmic_bytes(k, d, sp, sv):
+    """Per refined point: indices, neighbour coordinates and values, own coordinates, xi, output."""
+    return 4 * k + k * d * sp + k * sv + d * sp + sv + sv
+
+
+def algorithmic_flops(k, d):
+    """Per refined point: pair distances, interpolation, Cholesky (k^3/3), two solves, the rest."""
+    pairs = k * (k + 1) // 2
+    return pairs * (3 * d) + pairs * 3 + k ** 3 / 3 + 2 * k * k + 4 * k
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=16_777_216)
+    ap.add_argument("--k", type=int, default=30)
+    ap.add_argument("--n0", type=int, default=1000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--sample", type=int, default=2000, help="points checked against the oracle")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); every box in this setup has one GPU",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import graphgp_amd as gp
+    from graphgp_amd import _lib
+    import oracle as O
+
+    torch.cuda.set_device(0)
+    n, d, k, n0 = args.n, 3, args.k, args.n0
+    log(f"workload N={n} d={d} k={k} n0={n0}")
+    t = time.time()
+    pts = clustered(n, d, 64, 1e-4, 1e-1, seed=2024)
+    xi = np.random.default_rng(7).standard_normal(n).astype(np.float32)
+    log(f"inputs generated in {time.time() - t:.1f}s")
+    cov = gp.extras.matern_kernel(p=1, variance=1.0, cutoff=0.05, r_min=1e-6, r_max=2.0, n_bins=2001, jitter=1e-5)
+    pts_d = torch.from_numpy(pts).cuda()
+    xi_d = torch.from_numpy(xi).cuda()
+
+    torch.cuda.synchronize()
+    t = time.time()
+    g = gp.build_graph(pts_d, n0=n0, k=k)
+    torch.cuda.synchronize()
+    build_s = time.time() - t
+    log(f"build_graph {build_s * 1e3:.1f} ms, {len(g.offsets) - 1} batches")
+    n_ref = n - n0
+
+    start_ev = torch.cuda.Event(enable_timing=True)
+    stop_ev = torch.cuda.Event(enable_timing=True)
+    L = _lib.lib()
+
+    def timed(precision, steps, warmup):
+        out = None
+        for _ in range(warmup):
+            out = gp.generate(g, cov, xi_d, precision=precision)
+        torch.cuda.synchronize()
+        kms = []
+        t0 = time.time()
+        for _ in range(steps):
+            ks, ke = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            L.ggp_profile_events(ks.cuda_event, ke.cuda_event)
+            out = gp.generate(g, cov, xi_d, precision=precision)
+            L.ggp_profile_events(None, None)
+            kms.append((ks, ke))
+        torch.cuda.synchronize()
+        wall = (time.time() - t0) / steps
+        kernel_ms = float(np.mean([a.elapsed_time(b) for a, b in kms]))
+        return out, wall, kernel_ms
+
+    results = {}
+    for prec, steps in (("fp32_increment", args.steps), ("fp64", max(2, args.steps // 2)),
+                        ("fp32", max(2, args.steps // 2))):
+        out, wall, kms = timed(prec, steps, args.warmup)
+        results[prec] = dict(out=out, ms=wall * 1e3, kernel_ms=kms, pts_per_s=n_ref / wall)
+        log(f"{prec}: {wall * 1e3:.2f} ms/step, refine kernel {kms:.2f} ms, {n_ref / wall / 1e9:.3f} G pts/s")
+
+    # FP64 path checked point by point against the oracle (given its neighbours' FP64 values)
+    v64 = results["fp64"]["out"].cpu().numpy()
+    gpts = g.points.cpu().numpy()
+    gnb = g.neighbors.cpu().numpy()
+    gidx = g.indices.cpu().numpy()
+    vg = v64[gidx]  # graph order
+    xg = xi[gidx]
+    ocov = (cov[0].cpu().numpy(), cov[1].cpu().numpy())
+    rng = np.random.default_rng(1)
+    sample = rng.choice(n_ref, min(args.sample, n_ref), replace=False)
+    loc_err = 0.0
+    for r in sample:
+        mean, std = O.local_conditional(gpts, gnb[r], n0 + r, ocov, vg[gnb[r]])
+        loc_err = max(loc_err, abs(mean + std * float(xg[n0 + r]) - vg[n0 + r]))
+    errs = {}
+    for prec in ("fp32_increment", "fp32"):
+        dv = results[prec]["out"].cpu().numpy().astype(np.float64) - v64
+        errs[prec] = dict(max_abs=float(np.nanmax(np.abs(dv))), rms=float(np.sqrt(np.nanmean(dv ** 2))))
+    log(f"fp64 vs oracle (sampled, local): {loc_err:.3e}; errors vs fp64: {errs}")
+
+    # roofline of the FP32 refinement kernel (VALU bound: no MFMA in this kernel)
+    r32 = results["fp32_increment"]
+    kernel_s = r32["kernel_ms"] * 1e-3
+    flops = algorithmic_flops(k, d) * n_ref
+    bytes_ = algorithmic_bytes(k, d, 4, 4) * n_ref
+    tflops = flops / kernel_s / 1e12
+    gbs = bytes_ / kernel_s / 1e9
+    r64 = results["fp64"]
+    roofline = {
+        "bound": "valu",
+        "achieved": round(tflops, 3),
+        "peak": PEAK_FP32_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(tflops / PEAK_FP32_TFLOPS, 4),
+        "traffic": None,
+        "kernel": "refine_kernel<float,32,4,3> (fp32_increment)",
+        "kernel_ms": round(r32["kernel_ms"], 4),
+        "flop_per_point": algorithmic_flops(k, d),
+        "bytes_per_point": algorithmic_bytes(k, d, 4, 4),
+        "hbm_achieved_GBs": round(gbs, 1),
+        "hbm_frac": round(gbs / PEAK_HBM_GBS, 4),
+        "fp64_kernel_ms": round(r64["kernel_ms"], 4),
+        "fp64_tflops": round(algorithmic_flops(k, d) * n_ref / (r64["kernel_ms"] * 1e-3) / 1e12, 3),
+        "fp64_frac": round(algorithmic_flops(k, d) * n_ref / (r64["kernel_ms"] * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, 4),
+    }
+
+    cpu_baseline = None
+    if not args.no_cpu:
+        cores = len(os.sched_getaffinity(0))
+        cores = min(cores, 16)
+        O.set_threads(cores)
+        opts = gpts.astype(np.float64)
+        # calibrate, then a bounded sample of about args.cpu_seconds
+        m = min(20000, n_ref)
+        t = time.time()
+        O.refine(opts[: n0 + m], gnb[:m], g.offsets, ocov, vg[:n0], xg[n0:n0 + m])
+        rate = m / (time.time() - t)
+        m = int(min(n_ref, max(m, rate * args.cpu_seconds)))
+        t = time.time()
+        O.refine(opts[: n0 + m], gnb[:m], g.offsets, ocov, vg[:n0], xg[n0:n0 + m])
+        dt = time.time() - t
+        cpu_baseline = {
+            "value": round(m / dt, 1),
+            "unit": "refined points/s",
+            "cores": cores,
+            "kind": "port",
+            "sample": f"first {m} refined points of the same graph (FP64 oracle restatement of the fast_jit "
+                      f"refinement, refine.py:123-145), {dt:.1f} s",
+        }
+        log(f"cpu baseline: {m / dt:.0f} pts/s on {cores} threads")
+
+    line = {
+        "metric": "refined points/sec at N=16M,k=30 (FP32 vs plain FP64); p50/p99 serve latency",
+        "value": round(r32["pts_per_s"], 1),
+        "unit": "refined points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r32["ms"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": "generate, N=16,777,216 3-D 64-cluster mixture (widths 1e-4..1e-1), n0=1000, k=30, "
+                        "Matern-3/2 scale 0.05, 2001-entry table, jitter 1e-5, graph resident",
+            "n": n, "d": d, "k": k, "n0": n0, "n_batches": len(g.offsets) - 1,
+            "precision": "fp32_increment",
+        },
+        "fp64_points_per_s": round(r64["pts_per_s"], 1),
+        "fp64_ms_per_step": round(r64["ms"], 4),
+        "fp32_points_per_s": round(results["fp32"]["pts_per_s"], 1),
+        "speedup_fp32inc_vs_fp64": round(r32["pts_per_s"] / r64["pts_per_s"], 3),
+        "error_vs_fp64": errs,
+        "fp64_vs_oracle_sampled_max_abs": loc_err,
+        "build_graph_ms": round(build_s * 1e3, 1),
+        "roofline": roofline,
+        "cpu_baseline": cpu_baseline,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -50,6 +50,22 @@ __global__ void dense_cov_kernel(const void* __restrict__ pts, int pts_f64, int 
   A[e] = interp_dev(sqrt(s), bins, vals, nb);
 }
 
+__global__ void cross_cov_kernel(const void* __restrict__ pa, int64_t na, const void* __restrict__ pb, int64_t nb_,
+                                 int d, int f64, const double* __restrict__ bins, const double* __restrict__ vals,
+                                 int nbins, double* __restrict__ out) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= na * nb_) return;
+  int64_t i = e / nb_, j = e - i * nb_;
+  double s = 0.0;
+  for (int t = 0; t < d; ++t) {
+    double a = f64 ? ((const double*)pa)[i * d + t] : (double)((const float*)pa)[i * d + t];
+    double b = f64 ? ((const double*)pb)[j * d + t] : (double)((const float*)pb)[j * d + t];
+    double u = a - b;
+    s += u * u;
+  }
+  out[e] = interp_dev(sqrt(s), bins, vals, nbins);
+}
+
 // Factorise the diagonal block D (nbj x nbj, in LDS, lower) with one wavefront.
 __device__ inline void chol_lds(double (*Ds)[NB + 1], int nbj) {
   const int t = threadIdx.x;
@@ -231,6 +247,15 @@ int dense_apply(const double* L, int64_t n0, const double* xs, double* out, hipS
 int dense_solve(const double* L, int64_t n0, double* v, hipStream_t s) {
   if (n0 <= 0) return kOk;
   hipLaunchKernelGGL(trsv_kernel, dim3(1), dim3(1024), 0, s, L, n0, v);
+  GGP_LAUNCH_CHECK();
+  return kOk;
+}
+
+int cov_matrix(const void* a, int64_t na, const void* b, int64_t nb_, int d, int f64, const double* bins,
+               const double* vals, int nbins, double* out, hipStream_t s) {
+  if (na * nb_ == 0) return kOk;
+  hipLaunchKernelGGL(cross_cov_kernel, dim3(grid_for(na * nb_, 256)), dim3(256), 0, s, a, na, b, nb_, d, f64, bins,
+                     vals, nbins, out);
   GGP_LAUNCH_CHECK();
   return kOk;
 }

@@ -9,5 +9,7 @@ int dense_factor(const void* pts, int pts_f64, int d, int64_t n0, const double* 
                  double* A, hipStream_t s);
 int dense_apply(const double* L, int64_t n0, const double* xs, double* out, hipStream_t s);  // out = L xs
 int dense_solve(const double* L, int64_t n0, double* v, hipStream_t s);                     // v <- L^{-1} v
+int cov_matrix(const void* a, int64_t na, const void* b, int64_t nb, int d, int f64, const double* bins,
+               const double* vals, int nbins, double* out, hipStream_t s);
 int dense_logdiag(const double* L, int64_t n0, double* out, hipStream_t s);                 // sum log L_ii
 }  // namespace ggp

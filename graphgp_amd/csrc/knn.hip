@@ -56,11 +56,11 @@ template <typename T, int D, int KMAX>
 __global__ __launch_bounds__(256) void knn_kernel(const T* __restrict__ pts, const int32_t* __restrict__ split,
                                                   int d, const int32_t* __restrict__ qidx,
                                                   const int32_t* __restrict__ midx, int64_t q0, int64_t nq, int k,
-                                                  int32_t* __restrict__ out) {
+                                                  int32_t* __restrict__ out, int64_t n) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nq) return;
   const int64_t qi = qidx ? (int64_t)qidx[t] : q0 + t;
-  const int64_t max_index = midx ? (int64_t)midx[t] : qi;
+  const int64_t max_index = min(midx ? (int64_t)midx[t] : qi, n);
   Query<T, D> q;
 #pragma unroll
   for (int j = 0; j < D; ++j) q.c[j] = j < d ? pts[qi * d + j] : T(0);
@@ -143,11 +143,11 @@ __global__ __launch_bounds__(256) void knn_generic_kernel(const T* __restrict__ 
                                                           const int32_t* __restrict__ qidx,
                                                           const int32_t* __restrict__ midx, int64_t q0, int64_t nq,
                                                           int k, int32_t* __restrict__ out, T* __restrict__ wsd,
-                                                          int32_t* __restrict__ wsi) {
+                                                          int32_t* __restrict__ wsi, int64_t n) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nq) return;
   const int64_t qi = qidx ? (int64_t)qidx[t] : q0 + t;
-  const int64_t max_index = midx ? (int64_t)midx[t] : qi;
+  const int64_t max_index = min(midx ? (int64_t)midx[t] : qi, n);
   const T* q = pts + qi * d;
   T* sd = wsd + t * k;
   int32_t* si = wsi + t * k;
@@ -205,18 +205,18 @@ __global__ __launch_bounds__(256) void knn_generic_kernel(const T* __restrict__ 
 
 template <typename T, int D, int KMAX>
 void launch_knn(const T* pts, const int32_t* split, int d, const int32_t* qidx, const int32_t* midx, int64_t q0,
-                int64_t nq, int k, int32_t* out, hipStream_t s) {
+                int64_t nq, int k, int32_t* out, int64_t n, hipStream_t s) {
   hipLaunchKernelGGL((knn_kernel<T, D, KMAX>), dim3(grid_for(nq, 256)), dim3(256), 0, s, pts, split, d, qidx, midx,
-                     q0, nq, k, out);
+                     q0, nq, k, out, n);
 }
 
 template <typename T, int D>
 void dispatch_k(const T* pts, const int32_t* split, int d, const int32_t* qidx, const int32_t* midx, int64_t q0,
-                int64_t nq, int k, int32_t* out, hipStream_t s) {
-  if (k <= 8) launch_knn<T, D, 8>(pts, split, d, qidx, midx, q0, nq, k, out, s);
-  else if (k <= 16) launch_knn<T, D, 16>(pts, split, d, qidx, midx, q0, nq, k, out, s);
-  else if (k <= 32) launch_knn<T, D, 32>(pts, split, d, qidx, midx, q0, nq, k, out, s);
-  else launch_knn<T, D, 64>(pts, split, d, qidx, midx, q0, nq, k, out, s);
+                int64_t nq, int k, int32_t* out, int64_t n, hipStream_t s) {
+  if (k <= 8) launch_knn<T, D, 8>(pts, split, d, qidx, midx, q0, nq, k, out, n, s);
+  else if (k <= 16) launch_knn<T, D, 16>(pts, split, d, qidx, midx, q0, nq, k, out, n, s);
+  else if (k <= 32) launch_knn<T, D, 32>(pts, split, d, qidx, midx, q0, nq, k, out, n, s);
+  else launch_knn<T, D, 64>(pts, split, d, qidx, midx, q0, nq, k, out, n, s);
 }
 
 template <typename T>
@@ -235,15 +235,15 @@ int knn_impl(const T* pts, const int32_t* split, int64_t n, int d, const int32_t
       return kWorkspaceTooSmall;
     }
     hipLaunchKernelGGL(knn_generic_kernel<T>, dim3(grid_for(nq, 256)), dim3(256), 0, s, pts, split, d, qidx, midx,
-                       q0, nq, k, out, wsd, wsi);
+                       q0, nq, k, out, wsd, wsi, n);
   } else if (d == 1) {
-    dispatch_k<T, 1>(pts, split, d, qidx, midx, q0, nq, k, out, s);
+    dispatch_k<T, 1>(pts, split, d, qidx, midx, q0, nq, k, out, n, s);
   } else if (d == 2) {
-    dispatch_k<T, 2>(pts, split, d, qidx, midx, q0, nq, k, out, s);
+    dispatch_k<T, 2>(pts, split, d, qidx, midx, q0, nq, k, out, n, s);
   } else if (d == 3) {
-    dispatch_k<T, 3>(pts, split, d, qidx, midx, q0, nq, k, out, s);
+    dispatch_k<T, 3>(pts, split, d, qidx, midx, q0, nq, k, out, n, s);
   } else {
-    dispatch_k<T, 8>(pts, split, d, qidx, midx, q0, nq, k, out, s);
+    dispatch_k<T, 8>(pts, split, d, qidx, midx, q0, nq, k, out, n, s);
   }
   GGP_LAUNCH_CHECK();
   return kOk;

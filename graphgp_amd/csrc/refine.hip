@@ -236,12 +236,20 @@ __global__ __launch_bounds__(64) void refine_generic_kernel(RefineArgs<T> a, T* 
   }
 }
 
-constexpr int kGenericBlocks = 256;
+// Blocks of the generic kernel: at most 256, and at most 512 MB of slabs.
+template <typename T>
+int generic_blocks(int k) {
+  const size_t m = (size_t)k + 1;
+  const size_t per_block = (size_t)64 * (m * m + m) * sizeof(T);
+  const size_t cap = (size_t)512 << 20;
+  size_t b = cap / per_block;
+  return (int)std::max<size_t>(1, std::min<size_t>(256, b));
+}
 
 template <typename T>
 size_t generic_scratch_bytes(int k) {
   const size_t m = (size_t)k + 1;
-  return (size_t)kGenericBlocks * 64 * (m * m + m) * sizeof(T);
+  return (size_t)generic_blocks<T>(k) * 64 * (m * m + m) * sizeof(T);
 }
 
 // ------------------------------------------------------------------ small kernels
@@ -366,8 +374,27 @@ bool carve(Workspace& w, const RefineProblem& p, int mode, RefineWs<T>& b) {
   return w.ok();
 }
 
+}  // namespace
+
+// Optional events recorded around the refinement kernel (profiling hook, see
+// ggp_profile_events); null when unused.
+thread_local hipEvent_t g_prof_start = nullptr, g_prof_stop = nullptr;
+
+namespace {
+
+template <typename T>
+int run_kernel_inner(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hipStream_t s);
+
 template <typename T>
 int run_kernel(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hipStream_t s) {
+  if (g_prof_start) GGP_HIP(hipEventRecord(g_prof_start, s));
+  int rc = run_kernel_inner<T>(a, p, b, s);
+  if (g_prof_stop) GGP_HIP(hipEventRecord(g_prof_stop, s));
+  return rc;
+}
+
+template <typename T>
+int run_kernel_inner(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hipStream_t s) {
   a.pts = p.pts;
   a.pts_f64 = p.pts_f64;
   a.d = p.d;
@@ -386,7 +413,7 @@ int run_kernel(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hipStrea
   const bool generic = kk > 40 || p.d > 8 || (p.d > 3 && kk > 16);
   if (generic) {
     a.ntiles = (p.n_ref + 63) / 64;
-    const int grid = (int)std::min<int64_t>(a.ntiles, kGenericBlocks);
+    const int grid = (int)std::min<int64_t>(a.ntiles, generic_blocks<T>(p.k));
     hipLaunchKernelGGL(refine_generic_kernel<T>, dim3(grid), dim3(64), 0, s, a, b.scratch);
     GGP_LAUNCH_CHECK();
     return kOk;

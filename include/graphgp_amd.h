@@ -153,6 +153,17 @@ int ggp_generate_dense_logdet(const void* points, int pdtype, int64_t n, int d, 
                               const double* cov_vals, int n_bins, double* logdet_device, void* workspace,
                               size_t workspace_bytes, void* stream);
 
+/* ---- covariance matrix of two point sets: graphgp.compute_cov_matrix(covariance, a, b),
+ * refine.py:288-305 (r = ||a - b||, jnp.interp on the table).  out (na, nb) FP64. */
+int ggp_cov_matrix(const void* points_a, int64_t na, const void* points_b, int64_t nb, int d, int pdtype,
+                   const double* cov_bins, const double* cov_vals, int n_bins, double* out, void* stream);
+
+/* ---- profiling hook (no reference counterpart): when both are non-NULL hipEvent_t,
+ * every refinement flow on this host thread records start_event just before and
+ * stop_event just after its refinement kernel, on the call's stream.  Pass NULLs to
+ * switch off. */
+int ggp_profile_events(void* start_event, void* stop_event);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,298 @@
+"""Parity of the gfx950 kernels (through the C ABI) with the oracle and with the reference's
+own known answers.  Integer outputs (tree permutation, split dims, neighbours, depths,
+order, offsets) must match bit for bit; floating-point outputs within the tolerance
+stated in each test."""
+import numpy as np
+import pytest
+import torch
+
+import graphgp_amd as gp
+import oracle as O
+from helpers import clustered_points, ref_points
+
+pytestmark = pytest.mark.gpu
+
+
+def cpu(t):
+    return t.detach().cpu().numpy()
+
+
+# ---------------------------------------------------------------- tree
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_build_tree_simple(dtype):
+    # tests/test_tree.py:19-24
+    pts, sd, idx = gp.build_tree(np.arange(1, 11, dtype=dtype)[:, None])
+    assert np.array_equal(cpu(pts)[:, 0], [6, 3, 9, 2, 8, 5, 10, 1, 7, 4])
+    assert np.array_equal(cpu(sd), np.zeros(10))
+    assert np.array_equal(cpu(idx), [5, 2, 8, 1, 7, 4, 9, 0, 6, 3])
+
+
+def test_build_tree_random_reference_kat():
+    # tests/test_tree.py:27-32, whole arrays against the oracle
+    p = ref_points((100000, 3))
+    pts, sd, idx = gp.build_tree(p)
+    assert np.array_equal(cpu(sd)[1000:1005], [2, 0, 0, 1, 1])
+    assert int(cpu(idx)[-1]) == 59735
+    op, osd, oidx = O.build_tree(p)
+    assert np.array_equal(cpu(idx), oidx)
+    assert np.array_equal(cpu(sd), osd)
+    assert np.array_equal(cpu(pts), op)
+
+
+@pytest.mark.parametrize("n,d,seed", [(1, 3, 0), (2, 2, 1), (3, 1, 2), (17, 3, 3), (1000, 6, 4), (65537, 3, 5),
+                                      (40000, 2, 6)])
+def test_build_tree_vs_oracle_f32_clustered(n, d, seed):
+    p = clustered_points(n, d=d, seed=seed)
+    pts, sd, idx = gp.build_tree(p)
+    op, osd, oidx = O.build_tree(p)
+    assert np.array_equal(cpu(idx), oidx)
+    assert np.array_equal(cpu(sd), osd)
+    assert np.array_equal(cpu(pts), op)
+
+
+def test_build_tree_duplicates_and_ties():
+    rng = np.random.default_rng(11)
+    p = rng.integers(0, 7, (20000, 3)).astype(np.float32)
+    p[::5] = -0.0  # signed zeros sort equal to +0.0
+    pts, sd, idx = gp.build_tree(p)
+    op, osd, oidx = O.build_tree(p)
+    assert np.array_equal(cpu(idx), oidx) and np.array_equal(cpu(sd), osd)
+
+
+# ---------------------------------------------------------------- kNN
+@pytest.mark.parametrize("k,n0,n,d,dtype", [(30, 30, 1000, 3, np.float64), (10, 100, 20000, 2, np.float32),
+                                            (30, 1000, 60000, 3, np.float32), (8, 8, 5000, 1, np.float64),
+                                            (16, 50, 8000, 6, np.float32), (70, 70, 1500, 3, np.float32)])
+def test_query_preceding_vs_oracle(k, n0, n, d, dtype):
+    p = clustered_points(n, d=d, seed=k + n0, dtype=dtype)
+    op, osd, _ = O.build_tree(p)
+    nb = gp.query_preceding_neighbors(op, osd, n0=n0, k=k)
+    onb = O.query_preceding_neighbors(op, osd, n0=n0, k=k)
+    assert np.array_equal(cpu(nb), onb)
+
+
+def test_query_preceding_bruteforce_reference():
+    # tests/test_tree.py:44-59
+    pts, sd, _ = gp.build_tree(ref_points((1000, 3)))
+    nb = cpu(gp.query_preceding_neighbors(pts, sd, n0=30, k=30))
+    P = cpu(pts)
+    dist = np.linalg.norm(P[:, None] - P[None], axis=-1)
+    i, j = np.indices(dist.shape)
+    dist[i <= j] = np.inf
+    assert np.array_equal(nb, np.argsort(dist, axis=-1, kind="stable")[30:, :30])
+
+
+def test_query_preceding_lattice_ties():
+    rng = np.random.default_rng(5)
+    p = rng.integers(0, 6, (3000, 2)).astype(np.float32)
+    op, osd, _ = O.build_tree(p)
+    nb = gp.query_preceding_neighbors(op, osd, n0=12, k=12)
+    assert np.array_equal(cpu(nb), O.query_preceding_neighbors(op, osd, n0=12, k=12))
+
+
+def test_query_neighbors_explicit_indices():
+    p = clustered_points(5000, d=3, seed=9)
+    op, osd, _ = O.build_tree(p)
+    rng = np.random.default_rng(0)
+    q = rng.integers(0, 5000, 700)
+    m = np.maximum(q, rng.integers(20, 5000, 700))
+    nb = gp.query_neighbors(op, osd, q, m, k=12)
+    assert np.array_equal(cpu(nb), O.query_neighbors(op, osd, q, m, k=12))
+
+
+def test_query_n0_less_than_k_raises():
+    pts, sd, _ = gp.build_tree(ref_points((100, 2)))
+    with pytest.raises(ValueError, match="n0 must be at least k"):
+        gp.query_preceding_neighbors(pts, sd, n0=5, k=10)
+
+
+# ---------------------------------------------------------------- graph
+def test_compute_depths_kat():
+    # tests/test_graph.py:40-52
+    nb = np.array([[0, 1, 2], [0, 1, 2], [1, 3, 4], [2, 4, 5]])
+    assert np.array_equal(cpu(gp.compute_depths(nb, n0=3)), [0, 0, 0, 1, 1, 2, 3])
+
+
+def test_order_by_depth_kat():
+    # tests/test_graph.py:55-82
+    points = np.array([0.0, 1.0, 2.0, 3.0, 4.0, 5.0])[:, None]
+    p, i, n, d = gp.order_by_depth(points, np.arange(6), np.array([[0, 1], [1, 2], [2, 3], [0, 1]]),
+                                   np.array([0, 0, 1, 2, 3, 1]))
+    assert np.array_equal(cpu(p)[:, 0], [0, 1, 2, 5, 3, 4])
+    assert np.array_equal(cpu(i), [0, 1, 2, 5, 3, 4])
+    assert np.array_equal(cpu(n), [[0, 1], [0, 1], [1, 2], [2, 4]])
+    assert np.array_equal(cpu(d), [0, 0, 1, 1, 2, 3])
+
+
+def test_compute_depths_rejects_malformed():
+    with pytest.raises(ValueError):
+        gp.compute_depths(np.array([[0, 4], [1, 2]]), n0=3)
+
+
+def test_build_graph_reference_kat():
+    # tests/test_graph.py:14-27, plus whole arrays against the oracle
+    p = ref_points((100000, 3))
+    g = gp.build_graph(p, n0=1000, k=10)
+    assert len(g.offsets) == 61
+    assert g.offsets[1] == 1115
+    assert int(g.neighbors[0, 0]) == 488
+    assert np.isclose(float(g.points[0, 0]), -1.38417124, rtol=1e-8)
+    og = O.build_graph(p, n0=1000, k=10)
+    assert g.offsets == og.offsets
+    assert np.array_equal(cpu(g.neighbors), og.neighbors)
+    assert np.array_equal(cpu(g.indices), og.indices)
+    assert np.array_equal(cpu(g.points), og.points)
+    gp.check_graph(g)
+
+
+@pytest.mark.parametrize("n,d,n0,k", [(5000, 6, 500, 15), (50000, 3, 1000, 30), (20000, 2, 100, 10)])
+def test_build_graph_vs_oracle_f32(n, d, n0, k):
+    p = clustered_points(n, d=d, seed=n)
+    g = gp.build_graph(p, n0=n0, k=k)
+    og = O.build_graph(p, n0=n0, k=k)
+    assert g.offsets == og.offsets
+    assert g.offsets[0] == n0 and g.offsets[-1] == n
+    assert np.array_equal(cpu(g.neighbors), og.neighbors)
+    assert np.array_equal(cpu(g.indices), og.indices)
+    assert np.array_equal(cpu(g.points), og.points)
+
+
+def test_check_graph_detects_errors():
+    p = clustered_points(3000, d=2, seed=1)
+    g = gp.build_graph(p, n0=50, k=6)
+    gp.check_graph(g)
+    bad = g.neighbors.clone()
+    bad[-1, 0] = g.points.shape[0] - 1
+    with pytest.raises(AssertionError, match="topological"):
+        gp.check_graph(gp.Graph(g.points, bad, g.offsets, g.indices))
+    with pytest.raises(AssertionError, match="first offset"):
+        gp.check_graph(gp.Graph(g.points, g.neighbors, (49,) + g.offsets[1:], g.indices))
+
+
+# ---------------------------------------------------------------- refine (FP64)
+@pytest.fixture(scope="module")
+def ref_setup():
+    # tests/test_refine.py:15-26
+    points = ref_points((1000, 3))
+    og = O.build_graph(points, n0=100, k=10)
+    g = gp.build_graph(points, n0=100, k=10)
+    cov = gp.extras.matern_kernel(p=0, variance=1.0, cutoff=1.0, r_min=1e-4, r_max=10, n_bins=1000, jitter=1e-5)
+    ocov = O.matern_kernel(p=0, variance=1.0, cutoff=1.0, r_min=1e-4, r_max=10, n_bins=1000, jitter=1e-5)
+    return g, og, cov, ocov
+
+
+def test_logdet_reference_kat(ref_setup):
+    # tests/test_refine.py:29-43 (rtol 1e-8)
+    g, og, cov, _ = ref_setup
+    assert np.isclose(float(g.points[0, 0]), -1.95624711, rtol=1e-8)
+    assert np.isclose(float(gp.generate_logdet(g, cov, precision="fp64")), -600.36165088, rtol=1e-8)
+    assert np.isclose(float(gp.generate_dense_logdet(g.points, cov)), -610.90538067, rtol=1e-8)
+
+
+def test_generate_fp64_vs_oracle(ref_setup):
+    g, og, cov, ocov = ref_setup
+    xi = ref_points((1000,))
+    v = cpu(gp.generate(g, cov, xi))
+    ov = O.generate(og, ocov, xi)
+    assert v.dtype == np.float64
+    assert np.allclose(v, ov, rtol=1e-10, atol=1e-12)
+
+
+def test_inverse_round_trip_fp64(ref_setup):
+    # tests/test_refine.py:46-54
+    g, og, cov, ocov = ref_setup
+    xi = ref_points((1000,))
+    v = gp.generate(g, cov, xi)
+    xb = gp.generate_inv(g, cov, v)
+    assert np.allclose(cpu(xb), O.generate_inv(og, ocov, cpu(v)), rtol=1e-9, atol=1e-10)
+    assert np.allclose(cpu(gp.generate(g, cov, xb)), cpu(v), rtol=1e-12, atol=1e-13)
+
+
+def test_raw_refine_api_fp64(ref_setup):
+    g, og, cov, ocov = ref_setup
+    n0 = g.offsets[0]
+    xi = ref_points((1000,))[og.indices]
+    init = O.generate_dense(og.points[:n0], ocov, xi[:n0])
+    v = cpu(gp.refine(g.points, g.neighbors, g.offsets, cov, init, xi[n0:]))
+    ov = O.refine(og.points, og.neighbors, og.offsets, ocov, init, xi[n0:])
+    assert np.allclose(v, ov, rtol=1e-10, atol=1e-12)
+    i0, xr = gp.refine_inv(g.points, g.neighbors, g.offsets, cov, v)
+    assert np.allclose(cpu(i0), init, rtol=1e-14) and np.allclose(cpu(xr), xi[n0:], rtol=1e-8, atol=1e-9)
+    ld = float(gp.refine_logdet(g.points, g.neighbors, g.offsets, cov))
+    assert np.isclose(ld, O.refine_logdet(og.points, og.neighbors, og.offsets, ocov), rtol=1e-12)
+    assert np.allclose(cpu(gp.generate_dense(g.points[:n0], cov, xi[:n0])), init, rtol=1e-10, atol=1e-12)
+    assert np.allclose(cpu(gp.generate_dense_inv(g.points[:n0], cov, init)), xi[:n0], rtol=1e-8, atol=1e-9)
+    K = cpu(gp.compute_cov_matrix(cov, g.points[:50], g.points[:70]))
+    assert np.allclose(K, O.compute_cov_matrix(ocov, og.points[:50], og.points[:70]), rtol=1e-13)
+
+
+def test_approaches_dense_generic_k():
+    # tests/test_refine.py:66-80 (k = n0 = 200 runs the generic-k kernel)
+    points = ref_points((1000, 3))
+    g = gp.build_graph(points, n0=200, k=200)
+    g = gp.Graph(g.points, g.neighbors, g.offsets)
+    cov = gp.extras.matern_kernel(p=0, variance=1.0, cutoff=1.0, r_min=1e-4, r_max=10, n_bins=1000, jitter=1e-5)
+    xi = ref_points((1000,))
+    true_v = cpu(gp.generate_dense(g.points, cov, xi))
+    v = cpu(gp.generate(g, cov, xi))
+    assert np.allclose(true_v, v, atol=0.02)
+    ocov = O.matern_kernel(p=0, variance=1.0, cutoff=1.0, r_min=1e-4, r_max=10, n_bins=1000, jitter=1e-5)
+    og = O.OGraph(cpu(g.points), cpu(g.neighbors), g.offsets, None)
+    assert np.allclose(v, O.generate(og, ocov, xi), rtol=1e-9, atol=1e-10)
+
+
+# ---------------------------------------------------------------- refine (FP32)
+@pytest.fixture(scope="module")
+def c1_setup():
+    # the 2-D plain-method workload at 1/5 size: N(0,1) points, RBF var 1 scale 0.3, jitter 1e-4
+    rng = np.random.default_rng(99)
+    p = rng.standard_normal((20000, 2)).astype(np.float32)
+    g = gp.build_graph(p, n0=100, k=10)
+    og = O.OGraph(cpu(g.points), cpu(g.neighbors), g.offsets, cpu(g.indices))
+    cov = gp.extras.rbf_kernel(variance=1.0, scale=0.3, r_min=1e-4, r_max=10, n_bins=1001, jitter=1e-4)
+    ocov = O.rbf_kernel(variance=1.0, scale=0.3, r_min=1e-4, r_max=10, n_bins=1001, jitter=1e-4)
+    xi = rng.standard_normal(20000).astype(np.float32)
+    return g, og, cov, ocov, xi
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 2e-3), ("fp32_increment", 2e-4), ("fp64", 1e-10)])
+def test_generate_precisions_vs_fp64_oracle(c1_setup, precision, tol):
+    g, og, cov, ocov, xi = c1_setup
+    v = cpu(gp.generate(g, cov, xi, precision=precision))
+    ov = O.generate(og, ocov, xi)
+    err = np.abs(v - ov)
+    assert np.max(err) <= tol, (precision, float(np.max(err)), float(np.sqrt(np.mean(err ** 2))))
+
+
+@pytest.mark.parametrize("precision,tol", [("fp32", 2e-3), ("fp32_increment", 2e-4)])
+def test_inverse_and_logdet_fp32(c1_setup, precision, tol):
+    g, og, cov, ocov, xi = c1_setup
+    ov = O.generate(og, ocov, xi)
+    xb = cpu(gp.generate_inv(g, cov, ov.astype(np.float32), precision=precision))
+    oxb = O.generate_inv(og, ocov, ov.astype(np.float32).astype(np.float64))
+    assert np.max(np.abs(xb - oxb)) <= 50 * tol
+    ld = float(gp.generate_logdet(g, cov, precision=precision))
+    assert np.isclose(ld, O.generate_logdet(og, ocov), rtol=1e-5)
+
+
+def test_nan_rule_non_pd():
+    # duplicate points with zero jitter make the local covariance singular: all NaN
+    p = np.repeat(clustered_points(500, d=2, seed=2), 2, axis=0)
+    g = gp.build_graph(p, n0=10, k=5)
+    cov = gp.extras.rbf_kernel(variance=1.0, scale=0.3, r_min=1e-4, r_max=10, n_bins=100, jitter=0.0)
+    v = cpu(gp.generate(g, cov, np.ones(1000, np.float32)))
+    assert np.all(np.isnan(v))
+    x = cpu(gp.generate_inv(g, cov, np.ones(1000, np.float32)))
+    assert np.all(np.isnan(x))
+
+
+def test_argument_errors():
+    p = clustered_points(500, d=2, seed=3)
+    g = gp.build_graph(p, n0=20, k=5)
+    cov = gp.extras.rbf_kernel(variance=1.0, scale=0.3, r_min=1e-4, r_max=10, n_bins=100)
+    with pytest.raises(ValueError, match="Length of xi"):
+        gp.generate(g, cov, np.ones(499, np.float32))
+    with pytest.raises(ValueError, match="Invalid covariance"):
+        gp.generate(g, [cov[0], cov[1]], np.ones(500, np.float32))
+    with pytest.raises(ValueError, match="n0 must be at least k"):
+        gp.build_graph(p, n0=4, k=5)
