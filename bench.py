@@ -118,6 +118,8 @@ def main():
         t0 = time.time()
         for _ in range(steps):
             ks, ke = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            ks.record()  # creates the underlying hipEvents; the library re-records them
+            ke.record()  # around the refinement kernel on the same stream
             L.ggp_profile_events(ks.cuda_event, ke.cuda_event)
             out = gp.generate(g, cov, xi_d, precision=precision)
             L.ggp_profile_events(None, None)

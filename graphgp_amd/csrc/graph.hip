@@ -4,13 +4,12 @@
 // :95-96 (offsets), :44-70 (check_graph).
 //
 // compute_depths: depth[i] = 1 + max depth[neighbours(i)] (depth 0 for the n0 seed
-// points).  The reference iterates that map to its fixed point (one sweep per depth
-// level).  Here it is a single dataflow pass: every neighbour precedes its point, so
-// workgroups take tiles in increasing order from an atomic ticket and each lane
-// publishes its depth as soon as all its neighbours' depths are published (-1 marks
-// "not yet").  A lane only waits on lower tickets, which belong to workgroups that are
-// already running, so the pass always completes; a bounded spin flags a malformed
-// graph instead of hanging.  The result is the reference's unique fixed point.
+// points).  The reference iterates that map to its fixed point with Jacobi sweeps (one
+// sweep per depth level).  Here each sweep is Gauss-Seidel in index order (in place,
+// agent-coherent loads and stores, no waiting), so a dependency is usually resolved in
+// the sweep that reaches it and a handful of sweeps suffice; the host stops at the first
+// sweep that changes nothing, which is the reference's convergence test.  The fixed
+// point of a graph in topological order is unique, so the result is the reference's.
 //
 // order_by_depth: stable sort by depth (the radix sort of radix.hip), gathers, and the
 // neighbour remap through the inverse permutation.
@@ -24,65 +23,40 @@ namespace ggp {
 namespace {
 
 constexpr int kDepThreads = 256;
-constexpr uint32_t kSpinLimit = 1u << 24;
 
-__global__ __launch_bounds__(kDepThreads) void depths_kernel(const int32_t* __restrict__ nbrs, int64_t n_ref, int k,
-                                                            int64_t n0, int32_t* depths, uint32_t* ticket,
-                                                            uint32_t* err) {
-  __shared__ uint32_t s_tile;
-  const int64_t ntiles = (n_ref + kDepThreads - 1) / kDepThreads;
-  for (;;) {
-    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
-    __syncthreads();
-    const int64_t tile = s_tile;
-    __syncthreads();
-    if (tile >= ntiles) return;
-    const int64_t r = tile * kDepThreads + threadIdx.x;
+// One Gauss-Seidel sweep of depth[i] = 1 + max depth[nbrs(i)] in index order.  Values
+// written earlier in the same sweep may or may not be seen (no waiting, no ordering
+// assumption); a sweep that changes nothing has reached the reference's fixed point.
+// status bit 0: a neighbour that does not precede its point; bit 1: something changed.
+__global__ __launch_bounds__(kDepThreads) void depths_sweep_kernel(const int32_t* __restrict__ nbrs, int64_t n_ref,
+                                                                   int k, int64_t n0, int32_t* depths,
+                                                                   uint32_t* status) {
+  const int64_t r = (int64_t)blockIdx.x * kDepThreads + threadIdx.x;
+  bool changed = false, bad = false;
+  if (r < n_ref) {
     const int64_t i = n0 + r;
-    bool done = r >= n_ref;
-    if (!done) {
-      for (int j = 0; j < k; ++j) {
-        int32_t nb = nbrs[r * k + j];
-        if (nb < 0 || (int64_t)nb >= i) {
-          atomicOr(err, 1u);
-          __hip_atomic_store(&depths[i], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          done = true;
-          break;
-        }
-      }
-    }
-    for (uint32_t it = 0;; ++it) {
-      if (!done) {
-        int32_t m = -1;
-        bool ready = true;
-        for (int j = 0; j < k; ++j) {
-          int32_t nb = nbrs[r * k + j];
-          int32_t v = __hip_atomic_load(&depths[nb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          ready = ready && v >= 0;
-          m = v > m ? v : m;
-        }
-        if (ready) {
-          __hip_atomic_store(&depths[i], m + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          done = true;
-        }
-      }
-      if (__all(done)) break;
-      if (it >= kSpinLimit) {
-        if (!done) {
-          atomicOr(err, 2u);
-          __hip_atomic_store(&depths[i], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
+    int32_t m = -1;
+    for (int j = 0; j < k; ++j) {
+      const int32_t nb = nbrs[r * k + j];
+      if (nb < 0 || (int64_t)nb >= i) {
+        bad = true;
         break;
       }
-      __builtin_amdgcn_s_sleep(1);
+      const int32_t v = __hip_atomic_load(&depths[nb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      m = v > m ? v : m;
     }
-    __syncthreads();
+    if (!bad && m + 1 != depths[i]) {
+      __hip_atomic_store(&depths[i], m + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      changed = true;
+    }
   }
+  if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(status, 1u);
+  if (__any(changed) && (threadIdx.x & 63) == 0) atomicOr(status, 2u);
 }
 
-__global__ void depth_init_kernel(int32_t* depths, int64_t n, int64_t n0) {
+__global__ void depth_init_kernel(int32_t* depths, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) depths[i] = i < n0 ? 0 : -1;
+  if (i < n) depths[i] = 0;
 }
 
 __global__ void depth_keys_kernel(const int32_t* __restrict__ depths, int64_t n, uint32_t* __restrict__ keys,
@@ -165,6 +139,9 @@ __global__ void check_graph_kernel(const int32_t* __restrict__ nbrs, int64_t n_r
 
 size_t compute_depths_workspace() { return 512; }
 
+// Sweeps until nothing changes (synchronises once per sweep).  The number of sweeps is
+// at most max depth + 1 and in practice a handful: most neighbours lie far earlier in
+// the order and are final before the sweep reaches their dependents.
 int compute_depths(const int32_t* nbrs, int64_t n_ref, int k, int64_t n0, int32_t* depths, void* ws,
                    size_t ws_bytes, hipStream_t s) {
   GGP_CHECK_ARG(n0 >= 0 && n_ref >= 0 && k >= 1, "compute_depths: bad sizes");
@@ -176,16 +153,23 @@ int compute_depths(const int32_t* nbrs, int64_t n_ref, int k, int64_t n0, int32_
   }
   const int64_t n = n0 + n_ref;
   GGP_TRY(fill_u32(ctr, 0u, 2, s));
-  hipLaunchKernelGGL(depth_init_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, depths, n, n0);
+  hipLaunchKernelGGL(depth_init_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, depths, n);
   GGP_LAUNCH_CHECK();
   if (n_ref == 0) return kOk;
-  int dev = 0, cus = 256;
-  GGP_HIP(hipGetDevice(&dev));
-  GGP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  const int64_t ntiles = (n_ref + kDepThreads - 1) / kDepThreads;
-  const unsigned grid = (unsigned)std::min<int64_t>(ntiles, (int64_t)cus * 8);
-  hipLaunchKernelGGL(depths_kernel, dim3(grid), dim3(kDepThreads), 0, s, nbrs, n_ref, k, n0, depths, ctr, ctr + 1);
-  GGP_LAUNCH_CHECK();
+  uint32_t status = 0;
+  for (int64_t sweep = 0; sweep <= n_ref + 1; ++sweep) {
+    GGP_TRY(fill_u32(ctr, 0u, 1, s));
+    hipLaunchKernelGGL(depths_sweep_kernel, dim3(grid_for(n_ref, kDepThreads)), dim3(kDepThreads), 0, s, nbrs, n_ref,
+                       k, n0, depths, ctr);
+    GGP_LAUNCH_CHECK();
+    GGP_HIP(hipMemcpyAsync(&status, ctr, 4, hipMemcpyDeviceToHost, s));
+    GGP_HIP(hipStreamSynchronize(s));
+    if (status & 1u) break;      // malformed graph: report through the status word
+    if (!(status & 2u)) break;   // fixed point
+  }
+  uint32_t final_status = (status & 1u) ? 1u : 0u;
+  GGP_HIP(hipMemcpyAsync(ctr + 1, &final_status, 4, hipMemcpyHostToDevice, s));
+  GGP_HIP(hipStreamSynchronize(s));
   return kOk;
 }
 

@@ -332,21 +332,29 @@ __global__ __launch_bounds__(256) void refine_kernel(RefineArgs<T> a) {
       xi = load_as<T>(a.xi, a.xi_f64, dst - a.io_base);
     }
     bool done = !valid;
+    // neighbour values are fetched incrementally: a row is reloaded only while it still
+    // holds the "not yet published" sentinel, so a waiting lane issues few loads
+    T u[R];
+    uint32_t pend = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      u[r] = T(0);
+      if (valid && P * r + lane < k) pend |= 1u << r;
+    }
     for (uint32_t it = 0;; ++it) {
       if (!done) {
-        T u[R];
-        bool ok = true;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          const int i = P * r + lane;
-          u[r] = T(0);
-          if (i < k) {
-            const T v = __hip_atomic_load(&a.vals[a.nbrs[pt * k + i]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            ok = ok && !Sentinel<T>::is(v);
-            u[r] = v;
+          if (pend & (1u << r)) {
+            const T v = __hip_atomic_load(&a.vals[a.nbrs[pt * k + P * r + lane]], __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+            if (!Sentinel<T>::is(v)) {
+              u[r] = v;
+              pend &= ~(1u << r);
+            }
           }
         }
-        const uint64_t bal = __ballot(ok);
+        const uint64_t bal = __ballot(pend == 0);
         const int gbase = (threadIdx.x & 63) & ~(P - 1);
         const uint64_t gmask = (P == 64) ? ~0ull : ((1ull << P) - 1ull);
         if (((bal >> gbase) & gmask) == gmask) {
@@ -376,14 +384,15 @@ __global__ __launch_bounds__(256) void refine_kernel(RefineArgs<T> a) {
         }
       }
       if (__all(done)) break;
-      if (it > (1u << 24)) {
+      if (it > (1u << 22)) {
         if (!done) {
           atomicOr(&a.flags[1], 1u);
+          atomicOr(&a.flags[0], 1u);  // the whole output becomes NaN
           if (owner) __hip_atomic_store(&a.vals[fine], (T)__builtin_nan(""), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         break;
       }
-      __builtin_amdgcn_s_sleep(2);
+      __builtin_amdgcn_s_sleep(8);
     }
   }
 }
