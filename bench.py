@@ -154,6 +154,13 @@ def main():
     for prec in ("fp32_increment", "fp32"):
         dv = results[prec]["out"].cpu().numpy().astype(np.float64) - v64
         errs[prec] = dict(max_abs=float(np.nanmax(np.abs(dv))), rms=float(np.sqrt(np.nanmean(dv ** 2))))
+    # local (per-point) error of the FP32 paths: invert the FP64 values and compare with xi
+    v64_d = results["fp64"]["out"]
+    for prec in ("fp32_increment", "fp32"):
+        xb = gp.generate_inv(g, cov, v64_d.float(), precision=prec).cpu().numpy().astype(np.float64)
+        dx = np.abs(xb - xi)
+        errs[prec]["local_xi_max_abs"] = float(np.nanmax(dx))
+        errs[prec]["local_xi_rms"] = float(np.sqrt(np.nanmean(dx ** 2)))
     log(f"fp64 vs oracle (sampled, local): {loc_err:.3e}; errors vs fp64: {errs}")
 
     # roofline of the FP32 refinement kernel (VALU bound: no MFMA in this kernel)

@@ -83,7 +83,20 @@ __global__ void prep_meta_kernel(const double* __restrict__ bins, const double* 
     m.inv_step = 0.f;
     m.x1 = bins[nb > 1 ? 1 : 0];
   }
+  m.log_exact = m.inv_step > 0.f ? 1 : 0;
   *meta = m;
+}
+
+// Certifies the fast lookup: for every segment j >= 1 the guess at the geometric middle
+// of the segment must be j (as computed by the kernel, from r^2 in FP32).
+__global__ void prep_check_kernel(const double* __restrict__ bins, int nb, TabMeta* __restrict__ meta) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  if (j >= nb - 1) return;
+  const float lb1 = meta->lb1, inv_step = meta->inv_step;
+  const double mid = sqrt(bins[j] * bins[j + 1]);
+  int g = log_guess((float)(mid * mid), lb1, inv_step);
+  g = g < 1 ? 1 : (g > nb - 2 ? nb - 2 : g);
+  if (g != j || !(bins[j] > 0.0)) atomicAnd(&meta->log_exact, 0);
 }
 
 }  // namespace
@@ -100,6 +113,7 @@ int prep_table(const double* bins, const double* vals, int nb, int t64, int incr
     hipLaunchKernelGGL(prep_table_kernel<float>, dim3(grid_for(nb, 256)), dim3(256), 0, s, bins, vals, nb,
                        increment, (TabE<float>*)tab);
   hipLaunchKernelGGL(prep_meta_kernel, dim3(1), dim3(64), 0, s, bins, vals, nb, increment, meta);
+  if (nb > 2) hipLaunchKernelGGL(prep_check_kernel, dim3(grid_for(nb, 256)), dim3(256), 0, s, bins, nb, meta);
   GGP_LAUNCH_CHECK();
   return kOk;
 }

@@ -4,6 +4,11 @@
 
 namespace ggp {
 
+extern template int launch_config<float, 24, 4, 3>(RefineArgs<float>, hipStream_t, size_t, int);
+extern template int launch_config<float, 28, 4, 3>(RefineArgs<float>, hipStream_t, size_t, int);
+extern template int launch_config<float, 32, 4, 3>(RefineArgs<float>, hipStream_t, size_t, int);
+extern template int launch_config<float, 40, 8, 3>(RefineArgs<float>, hipStream_t, size_t, int);
+
 template <>
 int launch_refine<float>(const RefineArgs<float>& a, int kk, hipStream_t s, size_t lds, int grid) {
   if (a.d <= 3) {

@@ -4,6 +4,11 @@
 
 namespace ggp {
 
+extern template int launch_config<double, 20, 4, 3>(RefineArgs<double>, hipStream_t, size_t, int);
+extern template int launch_config<double, 24, 4, 3>(RefineArgs<double>, hipStream_t, size_t, int);
+extern template int launch_config<double, 32, 8, 3>(RefineArgs<double>, hipStream_t, size_t, int);
+extern template int launch_config<double, 40, 8, 3>(RefineArgs<double>, hipStream_t, size_t, int);
+
 template <>
 int launch_refine<double>(const RefineArgs<double>& a, int kk, hipStream_t s, size_t lds, int grid) {
   if (a.d <= 3) {

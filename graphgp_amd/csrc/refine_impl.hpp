@@ -16,7 +16,15 @@ struct TabMeta {
   int nseg;
   double x_first, x_last, f_first, f_last, f_zero, x1;
   float lb1, inv_step;
+  int log_exact;  // 1: the log2 guess alone finds the segment (log-spaced bins)
 };
+
+// Segment guess used by the fast lookup; the table check in the prep kernel uses the
+// same expression, so "log_exact" certifies exactly what the kernel computes.
+__device__ __forceinline__ int log_guess(float r2, float lb1, float inv_step) {
+  const float lr = 0.5f * __builtin_amdgcn_logf(r2);
+  return 1 + (int)((lr - lb1) * inv_step);
+}
 
 enum RefineMode : int { kGenerate = 0, kInverse = 1, kLogdet = 2 };
 
@@ -97,7 +105,7 @@ struct Sentinel<double> {
 template <typename T>
 __device__ __forceinline__ T rsqrt_t(T x);
 template <>
-__device__ __forceinline__ float rsqrt_t<float>(float x) { return __frsqrt_rn(x); }
+__device__ __forceinline__ float rsqrt_t<float>(float x) { return __builtin_amdgcn_rsqf(x); }
 template <>
 __device__ __forceinline__ double rsqrt_t<double>(double x) { return 1.0 / sqrt(x); }
 
@@ -138,6 +146,50 @@ __device__ __forceinline__ T table_lookup(const TabE<T>* __restrict__ tab, int n
   return e.f0 + (r - e.x0) * e.s;
 }
 
+// Lookup for log-spaced bins: one LDS read.  The FP32 guess is off by one segment only
+// for r within ~1e-7 (relative) of a bin edge, where the neighbouring segment's line
+// differs by a second-order term far below FP32 rounding, so FP32 uses it as is; FP64
+// corrects the segment with one compare-and-reload.
+template <typename T>
+__device__ __forceinline__ T table_lookup_fast(const TabE<T>* __restrict__ tab, int nseg, T x1, T x_last, T f_last,
+                                               float lb1, float inv_step, T r, T r2) {
+  int j = log_guess((float)r2, lb1, inv_step);
+  j = j < 1 ? 1 : (j > nseg - 1 ? nseg - 1 : j);
+  j = (r < x1) ? 0 : j;
+  TabE<T> e = tab[j];
+  if constexpr (sizeof(T) == 8) {
+    // FP64 keeps the exact segment: the guess is at most one segment off
+    if (r < e.x0 || r >= e.x1) {
+      j += (r < e.x0) ? -1 : 1;
+      j = j < 0 ? 0 : (j > nseg - 1 ? nseg - 1 : j);
+      e = tab[j];
+    }
+  }
+  const T v = e.f0 + (r - e.x0) * e.s;
+  return (r >= x_last) ? f_last : v;
+}
+
+template <typename T>
+__device__ __forceinline__ T fast_sqrt(T x) {
+  return sqrt(x);
+}
+template <>
+__device__ __forceinline__ float fast_sqrt<float>(float x) {
+  return __builtin_amdgcn_sqrtf(x);
+}
+
+template <typename T, int D>
+__device__ __forceinline__ T pair_dist2(const T (&a)[D], const T (&b)[D]) {
+  T t = a[0] - b[0];
+  T s = t * t;
+#pragma unroll
+  for (int j = 1; j < D; ++j) {
+    T u = a[j] - b[j];
+    s += u * u;
+  }
+  return s;
+}
+
 template <typename T, int D>
 __device__ __forceinline__ T pair_dist(const T (&a)[D], const T (&b)[D]) {
   T t = a[0] - b[0];
@@ -151,7 +203,7 @@ __device__ __forceinline__ T pair_dist(const T (&a)[D], const T (&b)[D]) {
 }
 
 // ------------------------------------------------------------------ the kernel
-template <typename T, int KK, int P, int D>
+template <typename T, int KK, int P, int D, bool FAST>
 __global__ __launch_bounds__(256) void refine_kernel(RefineArgs<T> a) {
   static_assert(KK % P == 0, "KK must be a multiple of P");
   constexpr int R = KK / P;
@@ -163,12 +215,21 @@ __global__ __launch_bounds__(256) void refine_kernel(RefineArgs<T> a) {
   __shared__ double s_red[4];
 
   const TabMeta meta = *a.meta;
+  // two variants are launched; the one that does not match the table exits at once
+  if ((meta.log_exact != 0) != FAST) return;
   for (int j = threadIdx.x; j < meta.nseg; j += blockDim.x) tab[j] = a.tab[j];
   __syncthreads();
   const T x1 = (T)meta.x1, x_last = (T)meta.x_last, f_last = (T)meta.f_last, f_zero = (T)meta.f_zero;
   const int nseg = meta.nseg;
   const float lb1 = meta.lb1, inv_step = meta.inv_step;
-  auto lookup = [&](T r) { return table_lookup<T>(tab, nseg, x1, x_last, f_last, lb1, inv_step, r); };
+  auto lookup2 = [&](T r2) {
+    if constexpr (FAST) {
+      const T r = fast_sqrt<T>(r2);
+      return table_lookup_fast<T>(tab, nseg, x1, x_last, f_last, lb1, inv_step, r, r2);
+    } else {
+      return table_lookup<T>(tab, nseg, x1, x_last, f_last, lb1, inv_step, sqrt(r2));
+    }
+  };
 
   const int lane = threadIdx.x & (P - 1);
   const int grp = threadIdx.x / P;
@@ -201,7 +262,7 @@ __global__ __launch_bounds__(256) void refine_kernel(RefineArgs<T> a) {
 #pragma unroll
       for (int j = 0; j < D; ++j) c0[j] = bcast<P, 0>(c[0][j]);
 #pragma unroll
-      for (int r = 0; r < R; ++r) g0[r] = lookup(pair_dist<T, D>(c[r], c0));
+      for (int r = 0; r < R; ++r) g0[r] = lookup2(pair_dist2<T, D>(c[r], c0));
     }
 
     // ---- covariance entries (rows of this lane, all columns up to the slot width)
@@ -218,7 +279,7 @@ __global__ __launch_bounds__(256) void refine_kernel(RefineArgs<T> a) {
 #pragma unroll
         for (int r = jr; r < R; ++r) {
           const int i = P * r + lane;
-          T v = lookup(pair_dist<T, D>(c[r], cj));
+          T v = lookup2(pair_dist2<T, D>(c[r], cj));
           if (incr) {
             if (j == 0) v = (i == 0) ? f_zero : -g0[r];
             else v = (i == 0) ? -gj : (g0[r] + gj) - v;

@@ -4,13 +4,9 @@
 
 namespace ggp {
 
-template <typename T, int KK, int P, int D>
-int launch_config(RefineArgs<T> a, hipStream_t s, size_t lds_bytes, int grid) {
-  auto kern = refine_kernel<T, KK, P, D>;
-  constexpr int64_t gpt = 256 / P;
-  a.ntiles = (a.n_ref + gpt - 1) / gpt;
-  if (a.ntiles == 0) return kOk;
-  if ((int64_t)grid > a.ntiles) grid = (int)a.ntiles;
+template <typename T, int KK, int P, int D, bool FAST>
+int launch_variant(const RefineArgs<T>& a, hipStream_t s, size_t lds_bytes, int grid) {
+  auto kern = refine_kernel<T, KK, P, D, FAST>;
   if (lds_bytes > 65536) {
     hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     if (e != hipSuccess) {
@@ -21,6 +17,18 @@ int launch_config(RefineArgs<T> a, hipStream_t s, size_t lds_bytes, int grid) {
   hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds_bytes, s, a);
   GGP_LAUNCH_CHECK();
   return kOk;
+}
+
+// Launches the fast-lookup variant (log-spaced bins) and the exact variant; the table
+// check written by the prep kernel makes exactly one of them do the work.
+template <typename T, int KK, int P, int D>
+int launch_config(RefineArgs<T> a, hipStream_t s, size_t lds_bytes, int grid) {
+  constexpr int64_t gpt = 256 / P;
+  a.ntiles = (a.n_ref + gpt - 1) / gpt;
+  if (a.ntiles == 0) return kOk;
+  if ((int64_t)grid > a.ntiles) grid = (int)a.ntiles;
+  GGP_TRY((launch_variant<T, KK, P, D, true>(a, s, lds_bytes, grid)));
+  return launch_variant<T, KK, P, D, false>(a, s, lds_bytes, grid);
 }
 
 

@@ -264,15 +264,17 @@ def test_generate_precisions_vs_fp64_oracle(c1_setup, precision, tol):
     assert np.max(err) <= tol, (precision, float(np.max(err)), float(np.sqrt(np.mean(err ** 2))))
 
 
-@pytest.mark.parametrize("precision,tol", [("fp32", 2e-3), ("fp32_increment", 2e-4)])
-def test_inverse_and_logdet_fp32(c1_setup, precision, tol):
+# log-det tolerance: plain FP32 loses up to ~1e-3 relative in sigma where sigma^2 = c - b'A^-1 b
+# cancels (near-duplicate neighbours); the increment formulation keeps it to ~1e-6 relative.
+@pytest.mark.parametrize("precision,tol,ld_rtol", [("fp32", 2e-3, 1e-4), ("fp32_increment", 2e-4, 1e-5)])
+def test_inverse_and_logdet_fp32(c1_setup, precision, tol, ld_rtol):
     g, og, cov, ocov, xi = c1_setup
     ov = O.generate(og, ocov, xi)
     xb = cpu(gp.generate_inv(g, cov, ov.astype(np.float32), precision=precision))
     oxb = O.generate_inv(og, ocov, ov.astype(np.float32).astype(np.float64))
     assert np.max(np.abs(xb - oxb)) <= 50 * tol
     ld = float(gp.generate_logdet(g, cov, precision=precision))
-    assert np.isclose(ld, O.generate_logdet(og, ocov), rtol=1e-5)
+    assert np.isclose(ld, O.generate_logdet(og, ocov), rtol=ld_rtol)
 
 
 def test_nan_rule_non_pd():
