@@ -154,6 +154,9 @@ def main():
     for prec in ("fp32_increment", "fp32"):
         dv = results[prec]["out"].cpu().numpy().astype(np.float64) - v64
         errs[prec] = dict(max_abs=float(np.nanmax(np.abs(dv))), rms=float(np.sqrt(np.nanmean(dv ** 2))))
+    _, cnt = gp.generate(g, cov, xi_d, precision="fp32_increment", return_counters=True)
+    fix_frac = cnt["fp64_recomputed"] / n_ref
+    log(f"fp32_increment: {cnt['fp64_recomputed']} points recomputed in FP64 ({100 * fix_frac:.3f}%)")
     # local (per-point) error of the FP32 paths: invert the FP64 values and compare with xi
     v64_d = results["fp64"]["out"]
     for prec in ("fp32_increment", "fp32"):
@@ -238,6 +241,7 @@ def main():
         "fp32_points_per_s": round(results["fp32"]["pts_per_s"], 1),
         "speedup_fp32inc_vs_fp64": round(r32["pts_per_s"] / r64["pts_per_s"], 3),
         "error_vs_fp64": errs,
+        "fp32_increment_fp64_recomputed_fraction": fix_frac,
         "fp64_vs_oracle_sampled_max_abs": loc_err,
         "build_graph_ms": round(build_s * 1e3, 1),
         "roofline": roofline,

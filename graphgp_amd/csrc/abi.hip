@@ -360,6 +360,12 @@ int ggp_cov_matrix(const void* points_a, int64_t na, const void* points_b, int64
   return cov_matrix(points_a, na, points_b, nb, d, pdtype == GGP_F64, cov_bins, cov_vals, n_bins, out, S(stream));
 }
 
+int ggp_refine_counters(const void* workspace, uint32_t* host_counters, void* stream) {
+  GGP_HIP(hipMemcpyAsync(host_counters, workspace, 16, hipMemcpyDeviceToHost, S(stream)));
+  GGP_HIP(hipStreamSynchronize(S(stream)));
+  return kOk;
+}
+
 int ggp_profile_events(void* start_event, void* stop_event) {
   ggp::g_prof_start = static_cast<hipEvent_t>(start_event);
   ggp::g_prof_stop = static_cast<hipEvent_t>(stop_event);

@@ -5,11 +5,12 @@
 // slogdet(K)/2 = sum log diag L).  The seed block is always factorised in FP64.
 //
 // Kernels: covariance assembly (one thread per entry, binary-search interp on the FP64
-// table), a right-looking blocked Cholesky with 64-wide panels (panel kernel: the
-// diagonal block is factorised in LDS by one wavefront and every row block of the
-// panel is solved against it; update kernel: 64x64 LDS tiles of the trailing lower
-// triangle), a wavefront-per-row triangular matvec, a blocked forward substitution in
-// one workgroup, and a fixed-order log-diagonal sum.
+// table); a right-looking blocked Cholesky with 64-wide block columns, two launches per
+// block column: the panel as a 64x64 GEMM against the inverse of the diagonal factor,
+// and the trailing lower-triangle update in 64x64 LDS tiles whose diagonal-owner
+// workgroup also factorises and inverts the next diagonal block in LDS; a
+// wavefront-per-row triangular matvec, a blocked forward substitution in one
+// workgroup, and a fixed-order log-diagonal sum.
 #include "common.hpp"
 #include "dense.hpp"
 
@@ -66,67 +67,103 @@ __global__ void cross_cov_kernel(const void* __restrict__ pa, int64_t na, const 
   out[e] = interp_dev(sqrt(s), bins, vals, nbins);
 }
 
-// Factorise the diagonal block D (nbj x nbj, in LDS, lower) with one wavefront.
-__device__ inline void chol_lds(double (*Ds)[NB + 1], int nbj) {
+// Factorise the diagonal block D (nbj x nbj, lower, in LDS) with 256 threads and write
+// its inverse to Di (lower).  Four threads share a row of the trailing update.
+__device__ inline void factor_invert_lds(double (*D)[NB + 1], double (*Di)[NB + 1], int nbj) {
   const int t = threadIdx.x;
+  const int row = t & (NB - 1), lane4 = t >> 6;
   for (int c = 0; c < nbj; ++c) {
-    const double dc = sqrt(Ds[c][c]);
     __syncthreads();
-    if (t == c) Ds[c][c] = dc;
-    if (t > c && t < nbj) Ds[t][c] /= dc;
+    const double dc = sqrt(D[c][c]);
     __syncthreads();
-    if (t > c && t < nbj) {
-      const double lt = Ds[t][c];
-      for (int m = c + 1; m <= t; ++m) Ds[t][m] -= lt * Ds[m][c];
+    if (t < nbj && t > c) D[t][c] /= dc;
+    if (t == c) D[c][c] = dc;
+    __syncthreads();
+    if (row > c && row < nbj) {
+      const double l = D[row][c];
+      for (int m = c + 1 + lane4; m <= row; m += 4) D[row][m] -= l * D[m][c];
     }
-    __syncthreads();
   }
+  __syncthreads();
+  // inverse: thread j solves L x = e_j (x_i = 0 for i < j)
+  if (t < NB) {
+    const int j = t;
+    double x[NB];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      double s = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+      for (int m = 0; m < i; ++m) s -= D[i][m] * x[m];
+      x[i] = (i < nbj) ? s / D[i][i] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i) Di[i][j] = x[i];
+  }
+  __syncthreads();
 }
 
-// blockIdx.x = row block offset from the diagonal block jb (0 = the diagonal block).
-__global__ __launch_bounds__(64) void panel_kernel(double* __restrict__ A, int64_t n, int jb) {
-  __shared__ double Ds[NB][NB + 1];
+// Block jb's diagonal factor (first step only; later steps factor inside update).
+__global__ __launch_bounds__(256) void diag_factor_kernel(double* __restrict__ A, int64_t n, int jb,
+                                                          double* __restrict__ Linv) {
+  __shared__ double D[NB][NB + 1];
+  __shared__ double Di[NB][NB + 1];
   const int64_t j0 = (int64_t)jb * NB;
   const int nbj = (int)min<int64_t>(NB, n - j0);
-  const int t = threadIdx.x;
-  for (int r = 0; r < nbj; ++r)
-    if (t <= r) Ds[r][t] = A[(j0 + r) * n + j0 + t];
-  __syncthreads();
-  chol_lds(Ds, nbj);
-  const int ib = jb + blockIdx.x;
-  if (blockIdx.x == 0) {
-    for (int r = 0; r < nbj; ++r) {
-      if (t <= r) A[(j0 + r) * n + j0 + t] = Ds[r][t];
-      else if (t < nbj) A[(j0 + r) * n + j0 + t] = 0.0;  // clear the upper part of the block
-    }
-    return;
+  for (int q = threadIdx.x; q < NB * NB; q += 256) {
+    const int r = q / NB, c = q % NB;
+    D[r][c] = (r < nbj && c < nbj) ? A[(j0 + r) * n + j0 + c] : 0.0;
   }
-  const int64_t i = (int64_t)ib * NB + t;
-  if (i >= n) return;
-  double x[NB];
-  double* row = A + i * n + j0;
-#pragma unroll
-  for (int c = 0; c < NB; ++c) x[c] = c < nbj ? row[c] : 0.0;
-  // x L^T = a  ->  x_c = (a_c - sum_{m<c} x_m L_cm) / L_cc
-#pragma unroll
-  for (int c = 0; c < NB; ++c) {
-    if (c < nbj) {
-      double s = x[c];
-#pragma unroll
-      for (int m = 0; m < c; ++m) s -= x[m] * Ds[c][m];
-      x[c] = s / Ds[c][c];
-    }
+  factor_invert_lds(D, Di, nbj);
+  for (int q = threadIdx.x; q < NB * NB; q += 256) {
+    const int r = q / NB, c = q % NB;
+    if (r < nbj && c < nbj) A[(j0 + r) * n + j0 + c] = c <= r ? D[r][c] : 0.0;
+    Linv[q] = Di[r][c];
   }
-#pragma unroll
-  for (int c = 0; c < NB; ++c)
-    if (c < nbj) row[c] = x[c];
 }
 
-// Trailing update A[ib][mb] -= L[ib][jb] L[mb][jb]^T for jb < mb <= ib (lower blocks).
-__global__ __launch_bounds__(256) void update_kernel(double* __restrict__ A, int64_t n, int jb, int nrem) {
+// Panel: L[ib][jb] = A[ib][jb] L_jj^{-T}, one 64 x 64 block per workgroup (blockIdx.x + 1
+// blocks below the diagonal).
+__global__ __launch_bounds__(256) void panel_gemm_kernel(double* __restrict__ A, int64_t n, int jb,
+                                                         const double* __restrict__ Linv) {
+  __shared__ double X[NB][NB + 1];
+  __shared__ double Li[NB][NB + 1];
+  const int ib = jb + 1 + blockIdx.x;
+  const int64_t j0 = (int64_t)jb * NB, i0 = (int64_t)ib * NB;
+  const int nbj = (int)min<int64_t>(NB, n - j0);
+  const int t = threadIdx.x;
+  for (int q = t; q < NB * NB; q += 256) {
+    const int r = q / NB, c = q % NB;
+    X[r][c] = (i0 + r < n && c < nbj) ? A[(i0 + r) * n + j0 + c] : 0.0;
+    Li[r][c] = Linv[q];
+  }
+  __syncthreads();
+  const int tr = (t / 16) * 4, tc = (t % 16) * 4;
+  double acc[4][4] = {};
+  for (int m = 0; m < nbj; ++m) {
+    double xr[4], lc[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) { xr[u] = X[tr + u][m]; lc[u] = Li[tc + u][m]; }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[u][v] += xr[u] * lc[v];
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t r = i0 + tr + u;
+      const int c = tc + v;
+      if (r < n && c < nbj) A[r * n + j0 + c] = acc[u][v];
+    }
+}
+
+// Trailing update A[ib][mb] -= L[ib][jb] L[mb][jb]^T for jb < mb <= ib.  The workgroup
+// that owns the next diagonal block (jb+1, jb+1) also factorises and inverts it.
+__global__ __launch_bounds__(256) void update_kernel(double* __restrict__ A, int64_t n, int jb, int nrem,
+                                                     double* __restrict__ Linv) {
   __shared__ double Li[NB][NB + 1];
   __shared__ double Lm[NB][NB + 1];
-  // decode blockIdx.x -> (a, b) with 0 <= b <= a < nrem
   int e = blockIdx.x;
   int aa = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
   while ((aa + 1) * (aa + 2) / 2 <= e) ++aa;
@@ -153,13 +190,32 @@ __global__ __launch_bounds__(256) void update_kernel(double* __restrict__ A, int
 #pragma unroll
       for (int v = 0; v < 4; ++v) acc[u][v] += li[u] * lm[v];
   }
+  if (e != 0) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int64_t r = i0 + tr + u, c = m0 + tc + v;
+        if (r < n && c < n && c <= r) A[r * n + c] -= acc[u][v];
+      }
+    return;
+  }
+  // next diagonal block: update in LDS, factorise, invert, write back
+  const int nb2 = (int)min<int64_t>(NB, n - i0);
+  __syncthreads();
 #pragma unroll
   for (int u = 0; u < 4; ++u)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const int64_t r = i0 + tr + u, c = m0 + tc + v;
-      if (r < n && c < n && c <= r) A[r * n + c] -= acc[u][v];
+      const int r = tr + u, c = tc + v;
+      Li[r][c] = (r < nb2 && c < nb2 && c <= r) ? A[(i0 + r) * n + i0 + c] - acc[u][v] : 0.0;
     }
+  factor_invert_lds(Li, Lm, nb2);
+  for (int q = t; q < NB * NB; q += 256) {
+    const int r = q / NB, c = q % NB;
+    if (r < nb2 && c < nb2) A[(i0 + r) * n + i0 + c] = c <= r ? Li[r][c] : 0.0;
+    Linv[q] = Lm[r][c];
+  }
 }
 
 // values[i] = sum_{j<=i} L[i][j] xs[j]; one wavefront per row.
@@ -218,7 +274,7 @@ __global__ __launch_bounds__(1024) void logdiag_kernel(const double* __restrict_
 
 }  // namespace
 
-size_t dense_workspace(int64_t n0) { return (size_t)(n0 * n0) * sizeof(double) + 256; }
+size_t dense_workspace(int64_t n0) { return (size_t)(n0 * n0 + NB * NB) * sizeof(double) + 256; }
 
 int dense_factor(const void* pts, int pts_f64, int d, int64_t n0, const double* bins, const double* vals, int nb,
                  double* A, hipStream_t s) {
@@ -227,11 +283,13 @@ int dense_factor(const void* pts, int pts_f64, int d, int64_t n0, const double* 
                      vals, nb, A);
   GGP_LAUNCH_CHECK();
   const int nblk = (int)((n0 + NB - 1) / NB);
-  for (int jb = 0; jb < nblk; ++jb) {
-    hipLaunchKernelGGL(panel_kernel, dim3(nblk - jb), dim3(64), 0, s, A, n0, jb);
+  double* Linv = A + (size_t)n0 * n0;  // 64 x 64 scratch after the matrix
+  hipLaunchKernelGGL(diag_factor_kernel, dim3(1), dim3(256), 0, s, A, n0, 0, Linv);
+  GGP_LAUNCH_CHECK();
+  for (int jb = 0; jb < nblk - 1; ++jb) {
     const int nrem = nblk - jb - 1;
-    if (nrem > 0)
-      hipLaunchKernelGGL(update_kernel, dim3(nrem * (nrem + 1) / 2), dim3(256), 0, s, A, n0, jb, nrem);
+    hipLaunchKernelGGL(panel_gemm_kernel, dim3(nrem), dim3(256), 0, s, A, n0, jb, Linv);
+    hipLaunchKernelGGL(update_kernel, dim3(nrem * (nrem + 1) / 2), dim3(256), 0, s, A, n0, jb, nrem, Linv);
     GGP_LAUNCH_CHECK();
   }
   return kOk;

@@ -40,10 +40,13 @@ namespace ggp {
 namespace {
 
 // ---------------------------------------------------------------- table prep
+// Table layout: nb-1 segments {c, s} (value c + s r) followed by the nb edges (bins).
 template <typename T>
 __global__ void prep_table_kernel(const double* __restrict__ bins, const double* __restrict__ vals, int nb,
                                   int increment, TabE<T>* __restrict__ tab) {
   int j = blockIdx.x * blockDim.x + threadIdx.x;
+  T* edges = reinterpret_cast<T*>(tab + (nb - 1));
+  if (j < nb) edges[j] = (T)bins[j];
   if (j >= nb - 1) return;
   const double f0 = vals[0];
   double x0 = bins[j], x1 = bins[j + 1];
@@ -52,13 +55,12 @@ __global__ void prep_table_kernel(const double* __restrict__ bins, const double*
     fa = f0 - fa;
     fb = f0 - fb;
   }
+  // jnp.interp: fp[i-1] + (x - xp[i-1]) / dx * df, constant fp[i-1] when dx is ~0
   double dx = x1 - x0;
-  double s = (fabs(dx) <= 1e-300) ? 0.0 : (fb - fa) / dx;
+  double s = (fabs(dx) <= 4.930380657631324e-32) ? 0.0 : (fb - fa) / dx;
   TabE<T> e;
-  e.x0 = (T)x0;
-  e.f0 = (T)fa;
   e.s = (T)s;
-  e.x1 = (T)x1;
+  e.c = sizeof(T) == 4 ? (T)fa : (T)(fa - s * x0);
   tab[j] = e;
 }
 
@@ -101,7 +103,7 @@ __global__ void prep_check_kernel(const double* __restrict__ bins, int nb, TabMe
 
 }  // namespace
 
-size_t table_workspace(int nb, int t64) { return (size_t)(nb) * (t64 ? sizeof(TabE<double>) : sizeof(TabE<float>)) + 512; }
+size_t table_workspace(int nb, int t64) { return (size_t)nb * (t64 ? 24 : 12) + 512; }
 
 int prep_table(const double* bins, const double* vals, int nb, int t64, int increment, void* tab, TabMeta* meta,
                hipStream_t s) {
@@ -153,7 +155,8 @@ __global__ __launch_bounds__(64) void refine_generic_kernel(RefineArgs<T> a, T* 
       }
       return sqrt(s);
     };
-    auto look = [&](T r) { return table_lookup<T>(a.tab, meta.nseg, x1, x_last, f_last, meta.lb1, meta.inv_step, r); };
+    const T* gedges = reinterpret_cast<const T*>(a.tab + meta.nseg);
+    auto look = [&](T r) { return table_lookup<T>(a.tab, gedges, meta.nseg, x_last, f_last, r); };
     T skk = T(1);
     if (valid) {
       for (int i = 0; i < m; ++i) {
@@ -334,6 +337,16 @@ __global__ void logdet_sum_kernel(const double* __restrict__ partials, int64_t n
   }
 }
 
+constexpr double kDefaultFixTau = 1e-3;
+
+// Pivot threshold of the FP64 recomputation: a point is recomputed when a Cholesky pivot
+// (or the fine point's Schur complement) falls below tau times its original diagonal,
+// i.e. when FP32 would lose more than ~log10(1/tau) digits.  GGP_FIX_TAU overrides it.
+double fix_tau() {
+  const char* e = getenv("GGP_FIX_TAU");
+  return e ? atof(e) : kDefaultFixTau;
+}
+
 int device_cus() {
   static int cus = 0;
   if (cus == 0) {
@@ -362,7 +375,9 @@ template <typename T>
 struct RefineWs {
   TabE<T>* tab = nullptr;
   TabMeta* meta = nullptr;
-  uint32_t* ctr = nullptr;  // [0] ticket, [1] NaN flag, [2] error flag
+  TabE<double>* tab64 = nullptr;  // FP64 direct table for the FP64 recomputation (FP32 increment mode)
+  TabMeta* meta64 = nullptr;
+  uint32_t* ctr = nullptr;  // [0] ticket, [1] NaN flag, [2] error flag, [3] FP64-recomputed points
   double* partials = nullptr;
   T* vals = nullptr;  // graph-order values (N)
   double* dense = nullptr;
@@ -373,14 +388,20 @@ struct RefineWs {
 
 template <typename T>
 bool carve(Workspace& w, const RefineProblem& p, int mode, RefineWs<T>& b) {
-  b.tab = w.take<TabE<T>>(p.nb);
+  b.ctr = w.take<uint32_t>(4);  // first, so ggp_refine_counters finds it at offset 0
+  b.tab = w.take<TabE<T>>(p.nb);  // nb-1 segments + nb edges (of T) fit in nb entries + nb/2
+  w.take<T>(p.nb + 2);
   b.meta = w.take<TabMeta>(1);
-  b.ctr = w.take<uint32_t>(4);
+  if (sizeof(T) == 4 && p.increment) {
+    b.tab64 = w.take<TabE<double>>(p.nb);
+    w.take<double>(p.nb + 2);
+    b.meta64 = w.take<TabMeta>(1);
+  }
   const int64_t n0 = p.n - p.n_ref;
   b.partials = w.take<double>((p.n_ref + 31) / 32 + 1);
   if (mode != kLogdet || p.with_seed) b.vals = w.take<T>(p.n);
   if (p.with_seed) {
-    b.dense = w.take<double>((size_t)n0 * n0);
+    b.dense = w.take<double>((size_t)n0 * n0 + 4096);
     b.dvec = w.take<double>(n0 + 1);
     b.dout = w.take<double>(n0 + 4);
   }
@@ -422,6 +443,10 @@ int run_kernel_inner(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hi
   a.flags = b.ctr + 1;
   a.partials = b.partials;
   a.increment = p.increment;
+  a.tab64 = b.tab64;
+  a.meta64 = b.meta64;
+  a.fix_count = b.ctr + 3;
+  a.fix_tau = fix_tau();
   if (p.n_ref == 0) return kOk;
   const int kk = p.k + 1;
   const bool generic = kk > 40 || p.d > 8 || (p.d > 3 && kk > 16);
@@ -432,11 +457,8 @@ int run_kernel_inner(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hi
     GGP_LAUNCH_CHECK();
     return kOk;
   }
-  const size_t lds = (size_t)(p.nb - 1) * sizeof(TabE<T>);
-  if (lds > max_lds()) {
-    set_error("refine: covariance table does not fit in LDS (" + std::to_string(p.nb) + " bins)");
-    return kUnsupported;
-  }
+  const size_t lds = (size_t)(p.nb - 1) * sizeof(TabE<T>) + (size_t)p.nb * sizeof(T);
+  a.fix_lds_offset = (int)((lds + 15) & ~size_t(15));
   const int grid = device_cus() * 4;
   int rc = launch_refine<T>(a, kk, s, lds, grid);
   if (rc == kUnsupported && ggp::last_error()[0] == 0) set_error("refine: unsupported configuration");
@@ -458,6 +480,7 @@ int refine_flow(const RefineProblem& p, int mode, const RefineIO& io, void* ws, 
   const int64_t n0 = p.n - p.n_ref;
   GGP_TRY(fill_u32(b.ctr, 0u, 4, s));
   GGP_TRY(prep_table(p.bins, p.vals, p.nb, sizeof(T) == 8, p.increment, b.tab, b.meta, s));
+  if (b.tab64) GGP_TRY(prep_table(p.bins, p.vals, p.nb, 1, 0, b.tab64, b.meta64, s));
   RefineArgs<T> a{};
   a.mode = mode;
   a.perm = p.perm;
@@ -549,6 +572,8 @@ int refine_flow(const RefineProblem& p, int mode, const RefineIO& io, void* ws, 
 
 }  // namespace
 
+size_t refine_lds_limit() { return max_lds(); }
+
 int refine_points_per_tile(int t64, int kk, int d) {
   if (kk > 40 || d > 8 || (d > 3 && kk > 16)) return 64;
   if (!t64) {
@@ -585,7 +610,7 @@ int refine_run(const RefineProblem& p, int mode, int t64, const RefineIO& io, vo
 // Dense-only flows (generate_dense / generate_dense_inv / generate_dense_logdet).
 size_t dense_flow_workspace(int64_t n0) {
   Workspace w(nullptr, 0);
-  w.take<double>((size_t)n0 * n0);
+  w.take<double>((size_t)n0 * n0 + 4096);
   w.take<double>(n0 + 1);
   w.take<double>(n0 + 4);
   w.take<uint32_t>(4);
@@ -597,7 +622,7 @@ int dense_flow(const void* pts, int pts_f64, int d, int64_t n0, const double* bi
                hipStream_t s) {
   GGP_CHECK_ARG(n0 >= 1 && d >= 1 && nb >= 2, "dense: bad sizes");
   Workspace w(ws, ws_bytes);
-  double* A = w.take<double>((size_t)n0 * n0);
+  double* A = w.take<double>((size_t)n0 * n0 + 4096);
   double* v = w.take<double>(n0 + 1);
   double* o = w.take<double>(n0 + 4);
   uint32_t* flags = w.take<uint32_t>(4);

@@ -7,10 +7,19 @@
 
 namespace ggp {
 
+// Segment j of the interpolation table on [edge_j, edge_j+1).  FP32: c is the value at
+// edge_j and the segment is c + (r - edge_j) s (r - edge_j is exact, so entries keep their
+// relative precision); FP64: c is the intercept and the segment is c + s r.
 template <typename T>
-struct alignas(4 * sizeof(T)) TabE {
-  T x0, f0, s, x1;  // segment [x0, x1): value f0 + (r - x0) * s
+struct alignas(2 * sizeof(T)) TabE {
+  T c, s;
 };
+
+template <typename T>
+__device__ __forceinline__ T seg_value(const TabE<T>& e, T edge, T r) {
+  if constexpr (sizeof(T) == 4) return e.c + (r - edge) * e.s;
+  else return e.c + e.s * r;
+}
 
 struct TabMeta {
   int nseg;
@@ -51,6 +60,14 @@ struct RefineArgs {
   int increment;
   int64_t ntiles;
   int64_t io_base;  // subtracted from the caller-order index of xi (generate) / out (inverse)
+  // FP64 recomputation of ill-conditioned points (FP32 modes): FP64 direct table, the
+  // pivot threshold, a counter of recomputed points and the byte offset of the per-wave
+  // LDS slabs after the table.  tab64 == nullptr switches it off.
+  const TabE<double>* tab64;
+  const TabMeta* meta64;
+  double fix_tau;
+  uint32_t* fix_count;
+  int fix_lds_offset;
 };
 
 // ------------------------------------------------------------------ helpers
@@ -114,59 +131,46 @@ __device__ __forceinline__ T load_as(const void* p, int f64, int64_t i) {
   return f64 ? (T)((const double*)p)[i] : (T)((const float*)p)[i];
 }
 
-// Table lookup: interp(r, bins, vals) (refine.py:305) on the LDS segment table.
+// Exact table lookup: interp(r, bins, vals) (refine.py:305) with the segment found by
+// binary search over the edges (any monotone bins).
 template <typename T>
-__device__ __forceinline__ T table_lookup(const TabE<T>* __restrict__ tab, int nseg, T x1, T x_last, T f_last,
-                                          float lb1, float inv_step, T r) {
+__device__ __forceinline__ T table_lookup(const TabE<T>* __restrict__ tab, const T* __restrict__ edges, int nseg,
+                                          T x_last, T f_last, T r) {
   if (r >= x_last) return f_last;
-  int j;
-  if (r < x1) {
-    j = 0;
-  } else {
-    float lr = __log2f((float)r);
-    j = 1 + (int)((lr - lb1) * inv_step);
-    j = j < 1 ? 1 : (j > nseg - 1 ? nseg - 1 : j);
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (edges[mid] <= r) lo = mid;
+    else hi = mid - 1;
   }
-  TabE<T> e = tab[j];
-#pragma unroll 1
-  for (int it = 0; it < 3 && (r < e.x0 || r >= e.x1); ++it) {
-    j += (r < e.x0) ? -1 : 1;
-    j = j < 0 ? 0 : (j > nseg - 1 ? nseg - 1 : j);
-    e = tab[j];
-  }
-  if (r < e.x0 || r >= e.x1) {
-    int lo = 0, hi = nseg - 1;
-    while (lo < hi) {
-      int mid = (lo + hi + 1) >> 1;
-      if (tab[mid].x0 <= r) lo = mid;
-      else hi = mid - 1;
-    }
-    e = tab[lo];
-  }
-  return e.f0 + (r - e.x0) * e.s;
+  return seg_value<T>(tab[lo], edges[lo], r);
 }
 
-// Lookup for log-spaced bins: one LDS read.  The FP32 guess is off by one segment only
-// for r within ~1e-7 (relative) of a bin edge, where the neighbouring segment's line
-// differs by a second-order term far below FP32 rounding, so FP32 uses it as is; FP64
-// corrects the segment with one compare-and-reload.
+// Lookup for log-spaced bins.  FP32: the log2 guess alone picks the segment (an 8-byte and
+// a 4-byte LDS read); it is off by one only for r within ~1e-6 (relative) of a bin edge, where
+// the neighbouring segment's line differs by a second-order term below FP32 rounding.
+// FP64: the guess is biased low so the segment is j or j+1, chosen by comparing with the
+// edge between them; the three reads are independent of each other (no branch).
 template <typename T>
-__device__ __forceinline__ T table_lookup_fast(const TabE<T>* __restrict__ tab, int nseg, T x1, T x_last, T f_last,
-                                               float lb1, float inv_step, T r, T r2) {
-  int j = log_guess((float)r2, lb1, inv_step);
-  j = j < 1 ? 1 : (j > nseg - 1 ? nseg - 1 : j);
-  j = (r < x1) ? 0 : j;
-  TabE<T> e = tab[j];
-  if constexpr (sizeof(T) == 8) {
-    // FP64 keeps the exact segment: the guess is at most one segment off
-    if (r < e.x0 || r >= e.x1) {
-      j += (r < e.x0) ? -1 : 1;
-      j = j < 0 ? 0 : (j > nseg - 1 ? nseg - 1 : j);
-      e = tab[j];
-    }
+__device__ __forceinline__ T table_lookup_fast(const TabE<T>* __restrict__ tab, const T* __restrict__ edges,
+                                               int nseg, T x1, T x_last, T f_last, float lb1, float inv_step, T r,
+                                               T r2) {
+  if constexpr (sizeof(T) == 4) {
+    int j = log_guess((float)r2, lb1, inv_step);
+    j = j < 1 ? 1 : (j > nseg - 1 ? nseg - 1 : j);
+    j = (r < x1) ? 0 : j;
+    const T v = seg_value<T>(tab[j], edges[j], r);
+    return (r >= x_last) ? f_last : v;
+  } else {
+    int j = log_guess((float)r2 * 0.99999f, lb1, inv_step);
+    j = j < 1 ? 1 : (j > nseg - 1 ? nseg - 1 : j);
+    j = (r < x1) ? 0 : j;
+    const int j1 = j + 1 > nseg - 1 ? nseg - 1 : j + 1;
+    const TabE<T> e0 = tab[j], e1 = tab[j1];
+    const T b = edges[j + 1];
+    const T v = (r >= b) ? (e1.c + e1.s * r) : (e0.c + e0.s * r);
+    return (r >= x_last) ? f_last : v;
   }
-  const T v = e.f0 + (r - e.x0) * e.s;
-  return (r >= x_last) ? f_last : v;
 }
 
 template <typename T>
@@ -202,9 +206,103 @@ __device__ __forceinline__ T pair_dist(const T (&a)[D], const T (&b)[D]) {
   return sqrt(s);
 }
 
+// ------------------------------------------------------------------ FP64 recomputation
+// One wavefront recomputes one point's conditional mean and standard deviation in FP64:
+// lane i < k+1 owns row i of the joint covariance (neighbours first, fine point last) in
+// an LDS slab, the table lookup is the exact FP64 one, the factorisation is a
+// right-looking Cholesky with wavefront fences between its phases.  Used for the rare
+// points whose FP32 factorisation loses accuracy (small pivots or a small Schur
+// complement relative to the diagonal).  Runs before the point's neighbour values are
+// needed (it depends on the geometry only): returns sigma and, if want_w, writes the
+// FP64 weights w (mean = w . v[nbrs]) to wslot, so that only a dot product is left
+// on the dataflow's critical path.
+template <typename T, int D, bool FAST>
+__device__ __forceinline__ double fix64(const RefineArgs<T>& a, int64_t pt, bool want_w, double* slab, int ld,
+                                        double* wslot) {
+  const int lane = threadIdx.x & 63;
+  const int k = a.k, m = k + 1;
+  const int64_t fine = a.n0 + pt;
+  const TabMeta mt = *a.meta64;
+  const double* edges = reinterpret_cast<const double*>(a.tab64 + mt.nseg);
+  int64_t id = -1;
+  if (lane < k) id = a.nbrs[pt * k + lane];
+  else if (lane == k) id = fine;
+  double c[D];
+#pragma unroll
+  for (int t = 0; t < D; ++t) c[t] = (id >= 0 && t < a.d) ? load_as<double>(a.pts, a.pts_f64, id * a.d + t) : 0.0;
+  // pass 1: squared distances of this lane's row into the slab (no memory latency)
+  for (int j = 0; j < m; ++j) {
+    double cj[D];
+#pragma unroll
+    for (int t = 0; t < D; ++t) cj[t] = __shfl(c[t], j, 64);
+    if (lane >= j && lane < m) slab[lane * ld + j] = pair_dist2<double, D>(c, cj);
+  }
+  // pass 2: table lookups in batches of 8 independent (global, L2-resident) reads
+  constexpr int B = 8;
+  for (int j0 = 0; j0 < m; j0 += B) {
+    double v[B];
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+      const int j = j0 + q;
+      v[q] = 0.0;
+      if (j <= lane && lane < m) {
+        const double r2 = slab[lane * ld + j];
+        if constexpr (FAST)
+          v[q] = table_lookup_fast<double>(a.tab64, edges, mt.nseg, mt.x1, mt.x_last, mt.f_last, mt.lb1,
+                                           mt.inv_step, sqrt(r2), r2);
+        else
+          v[q] = table_lookup<double>(a.tab64, edges, mt.nseg, mt.x_last, mt.f_last, sqrt(r2));
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < B; ++q)
+      if (j0 + q <= lane && lane < m) slab[lane * ld + j0 + q] = v[q];
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  // right-looking Cholesky, lane i owns row i; after the scaled column is written, the
+  // trailing update of a row runs in batches of 8 (all loads of a batch before its stores)
+  for (int col = 0; col < k; ++col) {
+    const double lc = sqrt(slab[col * ld + col]);
+    const double lcol = (lane > col && lane < m) ? slab[lane * ld + col] / lc : 0.0;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (lane == col) slab[col * ld + col] = lc;
+    if (lane > col && lane < m) slab[lane * ld + col] = lcol;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    for (int mm0 = col + 1; mm0 < m; mm0 += 8) {
+      double lmc[8], row[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int mm = mm0 + q;
+        const bool act = mm <= lane && lane < m;
+        lmc[q] = act ? slab[mm * ld + col] : 0.0;
+        row[q] = act ? slab[lane * ld + mm] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int mm = mm0 + q;
+        if (mm <= lane && lane < m) slab[lane * ld + mm] = row[q] - lcol * lmc[q];
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
+  const double sigma = sqrt(slab[k * ld + k]);
+  if (want_w) {
+    // weights w = L_A^{-T} z with z = L[k, :k]: mean = w . v[nbrs]
+    double zj = (lane < k) ? slab[k * ld + lane] : 0.0;
+    for (int i = k - 1; i >= 0; --i) {
+      const double wi = __shfl(zj, i, 64) / slab[i * ld + i];
+      if (lane == i) zj = wi;
+      if (lane < i) zj -= slab[i * ld + lane] * wi;
+    }
+    if (lane < k) wslot[lane] = zj;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  return sigma;
+}
+
 // ------------------------------------------------------------------ the kernel
 template <typename T, int KK, int P, int D, bool FAST>
-__global__ __launch_bounds__(256) void refine_kernel(RefineArgs<T> a) {
+__global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
   static_assert(KK % P == 0, "KK must be a multiple of P");
   constexpr int R = KK / P;
   constexpr int S = P * R * (R + 1) / 2;
@@ -213,11 +311,17 @@ __global__ __launch_bounds__(256) void refine_kernel(RefineArgs<T> a) {
   TabE<T>* tab = reinterpret_cast<TabE<T>*>(smem_raw);
   __shared__ uint32_t s_tile;
   __shared__ double s_red[4];
+  __shared__ int s_nfix;
+  __shared__ int s_fixlist[GPT];
+  __shared__ double s_fixsig[GPT];
 
   const TabMeta meta = *a.meta;
   // two variants are launched; the one that does not match the table exits at once
   if ((meta.log_exact != 0) != FAST) return;
+  T* edges = reinterpret_cast<T*>(tab + meta.nseg);
+  const T* gedges = reinterpret_cast<const T*>(a.tab + meta.nseg);
   for (int j = threadIdx.x; j < meta.nseg; j += blockDim.x) tab[j] = a.tab[j];
+  for (int j = threadIdx.x; j <= meta.nseg; j += blockDim.x) edges[j] = gedges[j];
   __syncthreads();
   const T x1 = (T)meta.x1, x_last = (T)meta.x_last, f_last = (T)meta.f_last, f_zero = (T)meta.f_zero;
   const int nseg = meta.nseg;
@@ -225,9 +329,9 @@ __global__ __launch_bounds__(256) void refine_kernel(RefineArgs<T> a) {
   auto lookup2 = [&](T r2) {
     if constexpr (FAST) {
       const T r = fast_sqrt<T>(r2);
-      return table_lookup_fast<T>(tab, nseg, x1, x_last, f_last, lb1, inv_step, r, r2);
+      return table_lookup_fast<T>(tab, edges, nseg, x1, x_last, f_last, lb1, inv_step, r, r2);
     } else {
-      return table_lookup<T>(tab, nseg, x1, x_last, f_last, lb1, inv_step, sqrt(r2));
+      return table_lookup<T>(tab, edges, nseg, x_last, f_last, sqrt(r2));
     }
   };
 
@@ -235,6 +339,13 @@ __global__ __launch_bounds__(256) void refine_kernel(RefineArgs<T> a) {
   const int grp = threadIdx.x / P;
   const int k = a.k;
   const bool incr = a.increment != 0;
+  const bool fixing = a.tab64 != nullptr;
+  const T tau = (T)a.fix_tau;
+  double* slab = reinterpret_cast<double*>(smem_raw + a.fix_lds_offset) + (threadIdx.x >> 6) * (KK * (KK + 1));
+  // FP64 weights of the recomputed points: one KK-slot per group of the workgroup
+  double* wgrp = reinterpret_cast<double*>(smem_raw + a.fix_lds_offset) + 4 * (KK * (KK + 1)) + (threadIdx.x / P) * KK;
+  const int gbase = (threadIdx.x & 63) & ~(P - 1);
+  const uint64_t gmask = (P == 64) ? ~0ull : ((1ull << P) - 1ull);
 
   for (;;) {
     if (threadIdx.x == 0) s_tile = atomicAdd(a.ticket, 1u);
@@ -267,6 +378,9 @@ __global__ __launch_bounds__(256) void refine_kernel(RefineArgs<T> a) {
 
     // ---- covariance entries (rows of this lane, all columns up to the slot width)
     T m[S];
+    T d0[R];  // original diagonal of this lane's rows (for the conditioning test)
+#pragma unroll
+    for (int r = 0; r < R; ++r) d0[r] = T(1);
     static_for<0, KK>([&](auto jc) {
       constexpr int j = decltype(jc)::value;
       constexpr int js = j % P, jr = j / P;
@@ -285,6 +399,7 @@ __global__ __launch_bounds__(256) void refine_kernel(RefineArgs<T> a) {
             else v = (i == 0) ? -gj : (g0[r] + gj) - v;
           }
           m[P * r * (r + 1) / 2 + j] = v;
+          if (j >= P * r && j < P * r + P && i == j) d0[r] = v;
         }
       }
     });
@@ -293,11 +408,13 @@ __global__ __launch_bounds__(256) void refine_kernel(RefineArgs<T> a) {
     T rdl[R];  // 1 / L_ii for this lane's rows
 #pragma unroll
     for (int r = 0; r < R; ++r) rdl[r] = T(0);
+    bool bad = false;  // a pivot lost most of its diagonal: FP32 is not accurate here
     static_for<0, KK - 1>([&](auto cc) {
       constexpr int col = decltype(cc)::value;
       constexpr int cs = col / P, cl = col % P;
       if (col < k) {
         const T dg = bcast<P, cl>(m[P * cs * (cs + 1) / 2 + col]);
+        if (fixing) bad = bad || !(dg >= tau * bcast<P, cl>(d0[cs]));
         const T rs = rsqrt_t<T>(dg);
         if (lane == cl) rdl[cs] = rs;
 #pragma unroll
@@ -322,6 +439,46 @@ __global__ __launch_bounds__(256) void refine_kernel(RefineArgs<T> a) {
       for (int q = 0; q < P; ++q)
         if (lane == q && P * r + q == k) skk = m[P * r * (r + 1) / 2 + P * r + q];
     const bool owner = (lane == (k % P));
+    bool flagged = false;
+    if (fixing) {
+      T dk = T(1);
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (P * r + lane == k) dk = d0[r];
+      const uint64_t bb = __ballot(owner && !(skk >= tau * dk));
+      flagged = valid && (bad || ((bb >> gbase) & gmask) != 0);
+    }
+    // FP64 recomputation of the tile's flagged points before any waiting: the flagged
+    // groups are listed in LDS and the workgroup's four wavefronts take them in turn
+    double sig64 = 1.0;
+    if (fixing) {
+      if (threadIdx.x == 0) s_nfix = 0;
+      __syncthreads();
+      if (flagged && lane == 0) s_fixlist[atomicAdd(&s_nfix, 1)] = grp;
+      __syncthreads();
+      const int nfix = s_nfix;
+      for (int it = threadIdx.x >> 6; it < nfix; it += 4) {
+        const int g = s_fixlist[it];
+        double* wslot = reinterpret_cast<double*>(smem_raw + a.fix_lds_offset) + 4 * (KK * (KK + 1)) + g * KK;
+        const double sg = fix64<T, D, FAST>(a, tile * GPT + g, a.mode != kLogdet, slab, KK + 1, wslot);
+        if ((threadIdx.x & 63) == 0) {
+          s_fixsig[g] = sg;
+          atomicAdd(a.fix_count, 1u);
+        }
+      }
+      __syncthreads();
+      if (flagged) sig64 = s_fixsig[grp];
+    }
+    // FP64 mean of a recomputed point from this lane's rows' raw neighbour values
+    auto fixed_mean = [&](const T (&uv)[R]) -> double {
+      double part = 0.0;
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (P * r + lane < k) part += wgrp[P * r + lane] * (double)uv[r];
+#pragma unroll
+      for (int o = 1; o < P; o <<= 1) part += __shfl_xor(part, o, 64);
+      return part;
+    };
 
     // ---- forward substitution of the (transformed) values; returns row k's residual
     auto forward = [&](T (&u)[R]) -> T {
@@ -343,10 +500,8 @@ __global__ __launch_bounds__(256) void refine_kernel(RefineArgs<T> a) {
 
     if (a.mode == kLogdet) {
       double ld = 0.0;
-      if (valid && owner) {
-        ld = 0.5 * log((double)skk);
-        if (ld != ld) atomicOr(&a.flags[0], 1u);
-      }
+      if (valid && owner) ld = flagged ? log(sig64) : 0.5 * log((double)skk);
+      if (valid && owner && ld != ld) atomicOr(&a.flags[0], 1u);
       // block reduction in a fixed order (deterministic)
       for (int o = 32; o > 0; o >>= 1) ld += __shfl_xor(ld, o, 64);
       if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = ld;
@@ -366,6 +521,7 @@ __global__ __launch_bounds__(256) void refine_kernel(RefineArgs<T> a) {
         if (valid && i < k) u[r] = a.vals[a.nbrs[pt * k + i]];
         if (valid && i == k) u[r] = a.vals[fine];
       }
+      const double mean64 = flagged ? fixed_mean(u) : 0.0;
       if (incr) {
         v0 = bcast<P, 0>(u[0]);
 #pragma unroll
@@ -375,11 +531,12 @@ __global__ __launch_bounds__(256) void refine_kernel(RefineArgs<T> a) {
         }
       }
       const T uk = forward(u);
+      double xi = (double)(uk / sqrt(skk));
+      if (flagged && owner) xi = ((double)a.vals[fine] - mean64) / sig64;
       if (valid && owner) {
-        const T xi = uk / sqrt(skk);
         if (xi != xi) atomicOr(&a.flags[0], 1u);
         const int64_t dst = (a.perm ? (int64_t)a.perm[fine] : fine) - a.io_base;
-        if (a.out_f64) ((double*)a.out)[dst] = (double)xi;
+        if (a.out_f64) ((double*)a.out)[dst] = xi;
         else ((float*)a.out)[dst] = (float)xi;
       }
       continue;
@@ -402,13 +559,27 @@ __global__ __launch_bounds__(256) void refine_kernel(RefineArgs<T> a) {
       u[r] = T(0);
       if (valid && P * r + lane < k) pend |= 1u << r;
     }
+    auto publish = [&](T v) {
+      if (v != v) {
+        atomicOr(&a.flags[0], 1u);
+        v = __builtin_nan("");
+      }
+      __hip_atomic_store(&a.vals[fine], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (a.out) {
+        if (a.out_f64) ((double*)a.out)[dst] = (double)v;
+        else ((float*)a.out)[dst] = (float)v;
+      }
+    };
     for (uint32_t it = 0;; ++it) {
       if (!done) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
           if (pend & (1u << r)) {
-            const T v = __hip_atomic_load(&a.vals[a.nbrs[pt * k + P * r + lane]], __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_AGENT);
+            // a published value never changes, so a cached read that is not the sentinel
+            // is final; only the sentinel needs the coherent re-read
+            const int32_t nb = a.nbrs[pt * k + P * r + lane];
+            T v = a.vals[nb];
+            if (Sentinel<T>::is(v)) v = __hip_atomic_load(&a.vals[nb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (!Sentinel<T>::is(v)) {
               u[r] = v;
               pend &= ~(1u << r);
@@ -416,32 +587,25 @@ __global__ __launch_bounds__(256) void refine_kernel(RefineArgs<T> a) {
           }
         }
         const uint64_t bal = __ballot(pend == 0);
-        const int gbase = (threadIdx.x & 63) & ~(P - 1);
-        const uint64_t gmask = (P == 64) ? ~0ull : ((1ull << P) - 1ull);
         if (((bal >> gbase) & gmask) == gmask) {
-          T v0 = T(0);
-          if (incr) {
-            v0 = bcast<P, 0>(u[0]);
+          if (flagged) {
+            const double mean64 = fixed_mean(u);
+            if (owner) publish((T)(mean64 + sig64 * (double)xi));
+            done = true;
+          } else {
+            T v0 = T(0);
+            if (incr) {
+              v0 = bcast<P, 0>(u[0]);
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-              const int i = P * r + lane;
-              if (i != 0 && i < k) u[r] -= v0;
+              for (int r = 0; r < R; ++r) {
+                const int i = P * r + lane;
+                if (i != 0 && i < k) u[r] -= v0;
+              }
             }
+            const T uk = forward(u);
+            if (owner) publish(v0 - uk + sqrt(skk) * xi);
+            done = true;
           }
-          const T uk = forward(u);
-          if (owner) {
-            T v = v0 - uk + sqrt(skk) * xi;
-            if (v != v) {
-              atomicOr(&a.flags[0], 1u);
-              v = __builtin_nan("");
-            }
-            __hip_atomic_store(&a.vals[fine], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (a.out) {
-              if (a.out_f64) ((double*)a.out)[dst] = (double)v;
-              else ((float*)a.out)[dst] = (float)v;
-            }
-          }
-          done = true;
         }
       }
       if (__all(done)) break;

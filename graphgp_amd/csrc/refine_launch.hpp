@@ -4,6 +4,8 @@
 
 namespace ggp {
 
+size_t refine_lds_limit();
+
 template <typename T, int KK, int P, int D, bool FAST>
 int launch_variant(const RefineArgs<T>& a, hipStream_t s, size_t lds_bytes, int grid) {
   auto kern = refine_kernel<T, KK, P, D, FAST>;
@@ -27,6 +29,15 @@ int launch_config(RefineArgs<T> a, hipStream_t s, size_t lds_bytes, int grid) {
   a.ntiles = (a.n_ref + gpt - 1) / gpt;
   if (a.ntiles == 0) return kOk;
   if ((int64_t)grid > a.ntiles) grid = (int)a.ntiles;
+  // the FP64 recomputation needs one KK x (KK+1) FP64 slab per wavefront and KK FP64
+  // weights per point group after the table
+  if (a.tab64)
+    lds_bytes = (size_t)a.fix_lds_offset + (size_t)4 * KK * (KK + 1) * sizeof(double) +
+                (size_t)(256 / P) * KK * sizeof(double);
+  if (lds_bytes > refine_lds_limit()) {
+    set_error("refine: covariance table does not fit in LDS (" + std::to_string(lds_bytes) + " bytes)");
+    return kUnsupported;
+  }
   GGP_TRY((launch_variant<T, KK, P, D, true>(a, s, lds_bytes, grid)));
   return launch_variant<T, KK, P, D, false>(a, s, lds_bytes, grid);
 }

@@ -9,7 +9,9 @@ compatibility; there is one code path.
   * ``"fp64"``           -- FP64, the reference formulation;
   * ``"fp32"``           -- FP32, the reference formulation;
   * ``"fp32_increment"`` -- FP32 in increments relative to the nearest neighbour (keeps the
-                            relative precision of the covariance entries when points cluster).
+                            relative precision of the covariance entries when points cluster);
+                            points whose FP32 pivots or Schur complement fall below 1e-2 of
+                            their diagonal are recomputed in FP64 inside the same kernel.
 The default is ``"fp64"`` when the floating-point input (xi / values / points) is float64,
 else ``"fp32"``.  Outputs have the compute dtype.
 """
@@ -38,7 +40,17 @@ def _out_dtype(precision):
     return torch.float64 if precision == "fp64" else torch.float32
 
 
-def generate(graph: Graph, covariance, xi, *, cuda: bool = False, fast_jit: bool = True, precision=None):
+def _counters(ws):
+    """Counters of the refinement flow that used workspace ``ws`` (synchronises)."""
+    import ctypes
+
+    buf = (ctypes.c_uint32 * 4)()
+    _lib.check(_lib.lib().ggp_refine_counters(_lib.ptr(ws), buf, _lib.stream()), "refine counters")
+    return {"tiles": int(buf[0]), "nan": int(buf[1]), "malformed": int(buf[2]), "fp64_recomputed": int(buf[3])}
+
+
+def generate(graph: Graph, covariance, xi, *, cuda: bool = False, fast_jit: bool = True, precision=None,
+             return_counters: bool = False):
     """GP realisation: dense Cholesky of the seed block, then refinement (refine.py:21-56).
 
     ``xi`` (N,) in the caller's (original) order; returns values (N,) in that order.  A
@@ -62,10 +74,11 @@ def generate(graph: Graph, covariance, xi, *, cuda: bool = False, fast_jit: bool
                               _lib.ptr(bins), _lib.ptr(vals), bins.numel(), _lib.ptr(x), _lib.dtype_code(x),
                               _lib.ptr(out), _lib.dtype_code(out), _lib.PREC[prec], _lib.ptr(ws), ws.numel(),
                               _lib.stream()), "generate")
-    return out
+    return (out, _counters(ws)) if return_counters else out
 
 
-def generate_inv(graph: Graph, covariance, values, *, cuda: bool = False, precision=None):
+def generate_inv(graph: Graph, covariance, values, *, cuda: bool = False, precision=None,
+                 return_counters: bool = False):
     """Inverse of ``generate`` (refine.py:162-183): values (N,) -> xi (N,), caller order."""
     pts = as_points(graph.points)
     v = as_device(values)
@@ -85,7 +98,7 @@ def generate_inv(graph: Graph, covariance, values, *, cuda: bool = False, precis
                                   _lib.ptr(bins), _lib.ptr(vals), bins.numel(), _lib.ptr(v), _lib.dtype_code(v),
                                   _lib.ptr(out), _lib.dtype_code(out), _lib.PREC[prec], _lib.ptr(ws), ws.numel(),
                                   _lib.stream()), "generate_inv")
-    return out
+    return (out, _counters(ws)) if return_counters else out
 
 
 def generate_logdet(graph: Graph, covariance, *, cuda: bool = False, precision=None):

@@ -33,7 +33,8 @@ extern "C" {
 /* Precision of the refinement arithmetic. */
 #define GGP_PREC_FP64 0      /* FP64 local systems (the reference formulation)               */
 #define GGP_PREC_FP32 1      /* FP32 local systems (the reference formulation)               */
-#define GGP_PREC_FP32_INC 2  /* FP32 in increments relative to the nearest neighbour         */
+#define GGP_PREC_FP32_INC 2  /* FP32 in increments relative to the nearest neighbour, with FP64
+                                recomputation of the points whose FP32 pivots lose accuracy   */
 
 const char* ggp_last_error(void);
 int ggp_abi_version(void);
@@ -157,6 +158,11 @@ int ggp_generate_dense_logdet(const void* points, int pdtype, int64_t n, int d, 
  * refine.py:288-305 (r = ||a - b||, jnp.interp on the table).  out (na, nb) FP64. */
 int ggp_cov_matrix(const void* points_a, int64_t na, const void* points_b, int64_t nb, int d, int pdtype,
                    const double* cov_bins, const double* cov_vals, int n_bins, double* out, void* stream);
+
+/* ---- counters of the last refinement flow that used `workspace` (syncs): [0] tiles issued,
+ * [1] NaN produced, [2] malformed graph, [3] points recomputed in FP64 by the
+ * GGP_PREC_FP32_INC mode (no reference counterpart). */
+int ggp_refine_counters(const void* workspace, uint32_t* host_counters, void* stream);
 
 /* ---- profiling hook (no reference counterpart): when both are non-NULL hipEvent_t,
  * every refinement flow on this host thread records start_event just before and

@@ -298,3 +298,18 @@ def test_argument_errors():
         gp.generate(g, [cov[0], cov[1]], np.ones(500, np.float32))
     with pytest.raises(ValueError, match="n0 must be at least k"):
         gp.build_graph(p, n0=4, k=5)
+
+
+# ---------------------------------------------------------------- seed block (FP64)
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 130, 1000])
+def test_dense_seed_block_vs_oracle(n):
+    rng = np.random.default_rng(n)
+    p = rng.uniform(0, 1, (n, 3)).astype(np.float32)
+    cov = gp.extras.matern_kernel(p=1, variance=1.0, cutoff=0.05, r_min=1e-6, r_max=2.0, n_bins=2001, jitter=1e-5)
+    ocov = O.matern_kernel(p=1, variance=1.0, cutoff=0.05, r_min=1e-6, r_max=2.0, n_bins=2001, jitter=1e-5)
+    xi = rng.standard_normal(n)
+    v = cpu(gp.generate_dense(p, cov, xi))
+    ov = O.generate_dense(p, ocov, xi)
+    assert np.allclose(v, ov, rtol=1e-9, atol=1e-11)
+    assert np.allclose(cpu(gp.generate_dense_inv(p, cov, ov)), xi, rtol=1e-7, atol=1e-8)
+    assert np.isclose(float(gp.generate_dense_logdet(p, cov)), O.generate_dense_logdet(p, ocov), rtol=1e-11)
