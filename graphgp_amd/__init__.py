@@ -10,6 +10,7 @@ from .refine import (
     generate,
     generate_inv,
     generate_logdet,
+    generate_vjp,
     generate_dense,
     generate_dense_inv,
     generate_dense_logdet,
@@ -32,6 +33,7 @@ __all__ = [
     "generate",
     "generate_inv",
     "generate_logdet",
+    "generate_vjp",
     "generate_dense",
     "generate_dense_inv",
     "generate_dense_logdet",

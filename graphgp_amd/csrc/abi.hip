@@ -256,6 +256,35 @@ int ggp_generate(const void* points, int pdtype, int64_t n, int d, const int32_t
   return refine_run(p, 0, t64_of(precision), io, workspace, workspace_bytes, S(stream));
 }
 
+size_t ggp_generate_vjp_workspace(int64_t n, int d, int64_t n0, int k, int n_bins, int precision) {
+  RefineProblem p = problem(nullptr, 0, n, d, nullptr, n0, k, nullptr, nullptr, nullptr, n_bins, precision, 1);
+  return vjp_workspace(p, t64_of(precision));
+}
+
+int ggp_generate_vjp(const void* points, int pdtype, int64_t n, int d, const int32_t* neighbors, int64_t n0, int k,
+                     const int32_t* indices, const double* cov_bins, const double* cov_vals, int n_bins,
+                     const void* xi, int xi_dtype, const void* values, int values_dtype, const void* values_bar,
+                     int values_bar_dtype, void* xi_bar, int xi_bar_dtype, double* cov_vals_bar, int precision,
+                     void* workspace, size_t workspace_bytes, void* stream) {
+  GGP_TRY(check_refine_args(n, n0, k, precision, pdtype));
+  GGP_TRY(check_dtype(xi_dtype, "generate_vjp xi"));
+  GGP_TRY(check_dtype(values_dtype, "generate_vjp values"));
+  GGP_TRY(check_dtype(values_bar_dtype, "generate_vjp values_bar"));
+  GGP_TRY(check_dtype(xi_bar_dtype, "generate_vjp xi_bar"));
+  RefineProblem p = problem(points, pdtype, n, d, neighbors, n0, k, indices, cov_bins, cov_vals, n_bins, precision, 1);
+  VjpIO io{};
+  io.xi = xi;
+  io.xi_f64 = xi_dtype == GGP_F64;
+  io.values = values;
+  io.values_f64 = values_dtype == GGP_F64;
+  io.gvalues = values_bar;
+  io.g_f64 = values_bar_dtype == GGP_F64;
+  io.gxi = xi_bar;
+  io.gxi_f64 = xi_bar_dtype == GGP_F64;
+  io.gtab = cov_vals_bar;
+  return vjp_run(p, t64_of(precision), io, workspace, workspace_bytes, S(stream));
+}
+
 int ggp_generate_inv(const void* points, int pdtype, int64_t n, int d, const int32_t* neighbors, int64_t n0, int k,
                      const int32_t* indices, const double* cov_bins, const double* cov_vals, int n_bins,
                      const void* values, int values_dtype, void* xi, int xi_dtype, int precision, void* workspace,

@@ -307,7 +307,7 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
   constexpr int R = KK / P;
   constexpr int S = P * R * (R + 1) / 2;
   constexpr int GPT = 256 / P;
-  extern __shared__ char smem_raw[];
+  extern __shared__ __attribute__((aligned(16))) char smem_raw[];
   TabE<T>* tab = reinterpret_cast<TabE<T>*>(smem_raw);
   __shared__ uint32_t s_tile;
   __shared__ double s_red[4];

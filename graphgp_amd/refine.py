@@ -55,6 +55,8 @@ def generate(graph: Graph, covariance, xi, *, cuda: bool = False, fast_jit: bool
 
     ``xi`` (N,) in the caller's (original) order; returns values (N,) in that order.  A
     non-positive-definite local covariance makes every value NaN, as in the reference.
+    Differentiable with respect to ``xi`` and the covariance values (``covariance[1]``)
+    when either requires grad: the backward pass is the adjoint kernel (``generate_vjp``).
     """
     pts = as_points(graph.points)
     x = as_device(xi)
@@ -62,11 +64,22 @@ def generate(graph: Graph, covariance, xi, *, cuda: bool = False, fast_jit: bool
     if x.numel() != n:
         raise ValueError("Length of xi must match number of points in graph.")
     nb = as_index(graph.neighbors)
-    n_ref, k = nb.shape
-    n0 = n - n_ref
     bins, vals = as_covariance(covariance)
     prec = _precision(precision, x, pts)
     perm = as_index(graph.indices) if graph.indices is not None else None
+    if torch.is_grad_enabled() and (x.requires_grad or vals.requires_grad):
+        if return_counters:
+            raise ValueError("return_counters is not available on the differentiable path")
+        from .autograd import GenerateFunction
+
+        return GenerateFunction.apply(x, vals, pts, nb, perm, bins, prec)
+    return _generate_raw(pts, nb, perm, bins, vals, x, prec, return_counters)
+
+
+def _generate_raw(pts, nb, perm, bins, vals, x, prec, return_counters=False):
+    n, d = pts.shape
+    n_ref, k = nb.shape
+    n0 = n - n_ref
     out = torch.empty(n, dtype=_out_dtype(prec), device=pts.device)
     L = _lib.lib()
     ws = _lib.workspace(L.ggp_refine_workspace(n, d, n0, k, bins.numel(), _lib.PREC[prec], 1))
@@ -75,6 +88,44 @@ def generate(graph: Graph, covariance, xi, *, cuda: bool = False, fast_jit: bool
                               _lib.ptr(out), _lib.dtype_code(out), _lib.PREC[prec], _lib.ptr(ws), ws.numel(),
                               _lib.stream()), "generate")
     return (out, _counters(ws)) if return_counters else out
+
+
+def generate_vjp(graph: Graph, covariance, xi, values, values_bar, *, precision=None):
+    """Reverse mode of ``generate``: what JAX autodiff of the reference's ``generate``
+    (refine.py:21-56, 79-159) returns for the cotangent ``values_bar``.
+
+    ``xi`` and ``values`` are the forward pass's input and output (caller order).  Returns
+    ``(xi_bar, cov_vals_bar)``: xi_bar (N,) in the compute dtype, cov_vals_bar (n_bins,)
+    FP64.  The local factors are recomputed, not stored.
+    """
+    pts = as_points(graph.points)
+    x = as_device(xi)
+    v = as_device(values)
+    g = as_device(values_bar)
+    n, d = pts.shape
+    if x.numel() != n or v.numel() != n or g.numel() != n:
+        raise ValueError("Length of xi, values and values_bar must match number of points in graph.")
+    nb = as_index(graph.neighbors)
+    bins, vals = as_covariance(covariance)
+    prec = _precision(precision, x, pts)
+    perm = as_index(graph.indices) if graph.indices is not None else None
+    return _generate_vjp_raw(pts, nb, perm, bins, vals, x, v, g, prec)
+
+
+def _generate_vjp_raw(pts, nb, perm, bins, vals, x, v, g, prec):
+    n, d = pts.shape
+    n_ref, k = nb.shape
+    n0 = n - n_ref
+    gxi = torch.empty(n, dtype=_out_dtype(prec), device=pts.device)
+    gvals = torch.empty(bins.numel(), dtype=torch.float64, device=pts.device)
+    L = _lib.lib()
+    ws = _lib.workspace(L.ggp_generate_vjp_workspace(n, d, n0, k, bins.numel(), _lib.PREC[prec]))
+    _lib.check(L.ggp_generate_vjp(_lib.ptr(pts), _lib.dtype_code(pts), n, d, _lib.ptr(nb), n0, k, _lib.ptr(perm),
+                                  _lib.ptr(bins), _lib.ptr(vals), bins.numel(), _lib.ptr(x), _lib.dtype_code(x),
+                                  _lib.ptr(v), _lib.dtype_code(v), _lib.ptr(g), _lib.dtype_code(g), _lib.ptr(gxi),
+                                  _lib.dtype_code(gxi), _lib.ptr(gvals), _lib.PREC[prec], _lib.ptr(ws), ws.numel(),
+                                  _lib.stream()), "generate_vjp")
+    return gxi, gvals
 
 
 def generate_inv(graph: Graph, covariance, values, *, cuda: bool = False, precision=None,

@@ -121,6 +121,18 @@ int ggp_generate_logdet(const void* points, int pdtype, int64_t n, int d, const 
                         const double* cov_bins, const double* cov_vals, int n_bins, double* logdet_device,
                         int precision, void* workspace, size_t workspace_bytes, void* stream);
 
+/* generate_vjp: reverse mode of generate -- what JAX autodiff of graphgp.generate
+ * (refine.py:21-56, 79-159) returns for the cotangent values_bar (n, caller order):
+ * xi_bar (n, caller order, dtype xi_bar_dtype) and cov_vals_bar (n_bins, FP64, the cotangent
+ * of cov_vals).  xi and values are the forward pass's input and output.  precision
+ * GGP_PREC_FP64 works in FP64; the FP32 modes in FP32 (FP64 seed block). */
+size_t ggp_generate_vjp_workspace(int64_t n, int d, int64_t n0, int k, int n_bins, int precision);
+int ggp_generate_vjp(const void* points, int pdtype, int64_t n, int d, const int32_t* neighbors, int64_t n0, int k,
+                     const int32_t* indices, const double* cov_bins, const double* cov_vals, int n_bins,
+                     const void* xi, int xi_dtype, const void* values, int values_dtype, const void* values_bar,
+                     int values_bar_dtype, void* xi_bar, int xi_bar_dtype, double* cov_vals_bar, int precision,
+                     void* workspace, size_t workspace_bytes, void* stream);
+
 /* refine: graphgp_cuda.refine(points, neighbors, offsets, cov_bins, cov_vals, initial_values,
  * xi), refine.py:119.  initial_values (n0), xi (n - n0) -> values (n, graph order) with the
  * compute dtype (FP64 for GGP_PREC_FP64, FP32 otherwise). */

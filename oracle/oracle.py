@@ -390,25 +390,45 @@ def dense_vjp(points, covariance, xi, gvalues):
     # S = L^{-T} P L^{-1} by two triangular solves
     S = sla.solve_triangular(L.T, sla.solve_triangular(L.T, P, lower=False).T, lower=False).T
     Kbar = 0.5 * (S + S.T)
-    gvals = np.zeros(len(bins))
     r = np.sqrt(((p[:, None, :] - p[None, :, :]) ** 2).sum(-1))
-    for i in range(n):
-        for j in range(n):
-            _lookup_grad_acc(gvals, bins, r[i, j], Kbar[i, j])
-    return gxi, gvals
+    return gxi, _lookup_grad_all(bins, r.ravel(), Kbar.ravel())
 
 
-def _lookup_grad_acc(gvals, bins, r, g):
+def generate_vjp(graph, covariance, xi, gvalues):
+    """Reverse mode of generate (refine.py:21-56): refine_vjp over the refined points, then
+    dense_vjp of the seed block with the seeds' accumulated cotangents.  Caller order in and
+    out; returns (xi_bar (N,), cov_vals_bar (n_bins,))."""
+    n = len(graph.points)
+    n0 = n - len(graph.neighbors)
+    x = _f64(xi)
+    g = _f64(gvalues)
+    if graph.indices is not None:
+        x = x[graph.indices]
+        g = g[graph.indices]
+    init = generate_dense(graph.points[:n0], covariance, x[:n0])
+    v = refine(graph.points, graph.neighbors, graph.offsets, covariance, init, x[n0:])
+    gxr, ginit, gvr = refine_vjp(graph.points, graph.neighbors, covariance, v, x[n0:], g)
+    gxs, gvd = dense_vjp(graph.points[:n0], covariance, x[:n0], ginit)
+    gx = np.concatenate([gxs, gxr])
+    if graph.indices is not None:
+        out = np.empty_like(gx)
+        out[graph.indices] = gx
+        gx = out
+    return gx, gvr + gvd
+
+
+def _lookup_grad_all(bins, r, g):
+    """Cotangent of interp(r, bins, vals) w.r.t. vals, summed over arrays r with weights g:
+    (1 - f) to vals[i-1] and f to vals[i]; the clamps feed the end values."""
     nb = len(bins)
-    if r < bins[0]:
-        gvals[0] += g
-        return
-    if r > bins[-1]:
-        gvals[-1] += g
-        return
-    i = int(np.searchsorted(bins, r, side="right"))
-    i = min(max(i, 1), nb - 1)
-    dx = bins[i] - bins[i - 1]
-    f = (r - bins[i - 1]) / dx
-    gvals[i - 1] += g * (1 - f)
-    gvals[i] += g * f
+    gvals = np.zeros(nb)
+    lo, hi = r < bins[0], r > bins[-1]
+    gvals[0] += g[lo].sum()
+    gvals[-1] += g[hi].sum()
+    mid = ~(lo | hi)
+    rm, gm = r[mid], g[mid]
+    i = np.clip(np.searchsorted(bins, rm, side="right"), 1, nb - 1)
+    f = (rm - bins[i - 1]) / (bins[i] - bins[i - 1])
+    np.add.at(gvals, i - 1, gm * (1 - f))
+    np.add.at(gvals, i, gm * f)
+    return gvals

@@ -193,3 +193,26 @@ def test_dense_vjp_finite_differences():
         e[j] = 1e-7
         fd = (loss(xi, cov[1] + e) - loss(xi, cov[1] - e)) / 2e-7
         assert np.isclose(gvals[j], fd, rtol=1e-4, atol=1e-6)
+
+
+def test_generate_vjp_composition_pinned():
+    # generate is linear in xi, so <w, generate(u)> = <xi_bar(w), u> exactly (dot-product
+    # test); the table cotangent is pinned by central differences of the whole generate.
+    rng = np.random.default_rng(9)
+    p = rng.uniform(0, 1, (400, 3))
+    g = O.build_graph(p, n0=30, k=8)
+    cov = O.rbf_kernel(variance=1.0, scale=0.2, r_min=1e-4, r_max=2, n_bins=150, jitter=1e-3)
+    xi = rng.standard_normal(400)
+    w = rng.standard_normal(400)
+    gx, gv = O.generate_vjp(g, cov, xi, w)
+    for _ in range(3):
+        u = rng.standard_normal(400)
+        lhs = float(w @ O.generate(g, cov, u))
+        assert np.isclose(lhs, float(gx @ u), rtol=1e-10)
+    loss = lambda vals: float(w @ O.generate(g, (cov[0], vals), xi))  # noqa: E731
+    nz = np.nonzero(np.abs(gv) > 1e-3)[0]
+    for j in nz[:: max(1, len(nz) // 8)]:
+        e = np.zeros_like(cov[1])
+        e[j] = 1e-7
+        fd = (loss(cov[1] + e) - loss(cov[1] - e)) / 2e-7
+        assert np.isclose(gv[j], fd, rtol=1e-4, atol=1e-6)
