@@ -49,12 +49,19 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(verbose: bool = False, jobs: int | None = None) -> str:
+def build(verbose: bool = False, jobs: int | None = None, only=None) -> str:
+    """Compile every source (or, with ``only``, just those file names, reusing the other
+    objects of an earlier build) and link the library."""
     os.makedirs(OUT_DIR, exist_ok=True)
     srcs = sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
-    jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 4)))
+    todo = srcs if not only else [s for s in srcs if os.path.basename(s) in set(only)]
+    jobs = jobs or min(max(1, len(todo)), max(1, (os.cpu_count() or 4)))
     with cf.ThreadPoolExecutor(jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
+        list(ex.map(_compile, todo))
+    objs = [os.path.join(OUT_DIR, os.path.basename(s) + ".o") for s in srcs]
+    missing = [o for o in objs if not os.path.exists(o)]
+    if missing:
+        raise RuntimeError(f"objects missing (run a full build): {missing}")
     tmp = LIB + ".tmp"
     cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -67,5 +74,5 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, only=sys.argv[1:] or None)
     sys.exit(0)

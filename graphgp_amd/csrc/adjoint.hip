@@ -20,11 +20,12 @@ namespace ggp {
 template <>
 int launch_adjoint<float>(AdjointArgs<float> a, int kk, hipStream_t s, size_t lds, int grid) {
   if (a.d > 3) return kUnsupported;
+  // at most four rows per lane: the adjoint keeps w, z and the factor live together
   if (kk <= 8) return launch_adjoint_config<float, 8, 2, 3>(a, s, lds, grid);
-  if (kk <= 12) return launch_adjoint_config<float, 12, 2, 3>(a, s, lds, grid);
+  if (kk <= 12) return launch_adjoint_config<float, 12, 4, 3>(a, s, lds, grid);
   if (kk <= 16) return launch_adjoint_config<float, 16, 4, 3>(a, s, lds, grid);
-  if (kk <= 24) return launch_adjoint_config<float, 24, 4, 3>(a, s, lds, grid);
-  if (kk <= 32) return launch_adjoint_config<float, 32, 8, 3>(a, s, lds, grid);
+  if (kk <= 24) return launch_adjoint_config<float, 24, 8, 3>(a, s, lds, grid);
+  if (kk <= 32) return launch_adjoint_config<float, 32, 16, 3>(a, s, lds, grid);
   return kUnsupported;
 }
 
@@ -71,16 +72,24 @@ __global__ __launch_bounds__(256) void trmv_t_kernel(const double* __restrict__ 
   }
 }
 
-// X = L^{-1}; one thread per column
-__global__ void tri_inv_kernel(const double* __restrict__ L, int64_t n, double* __restrict__ X) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
-  for (int64_t i = 0; i < c; ++i) X[i * n + c] = 0.0;
+// X = L^{-1}; one wavefront per column c, the column held in LDS:
+// x_i = (delta_ic - sum_{m=c}^{i-1} L[i][m] x_m) / L[i][i], the sum split over the 64 lanes
+__global__ __launch_bounds__(64) void tri_inv_kernel(const double* __restrict__ L, int64_t n, double* __restrict__ X) {
+  extern __shared__ double xc[];
+  const int64_t c = blockIdx.x;
+  const int lane = threadIdx.x;
+  for (int64_t i = lane; i < c; i += 64) X[i * n + c] = 0.0;
   for (int64_t i = c; i < n; ++i) {
-    double s = (i == c) ? 1.0 : 0.0;
     const double* row = L + i * n;
-    for (int64_t m = c; m < i; ++m) s -= row[m] * X[m * n + c];
-    X[i * n + c] = s / row[i];
+    double s = 0.0;
+    for (int64_t m = c + lane; m < i; m += 64) s += row[m] * xc[m - c];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    const double xi = ((i == c ? 1.0 : 0.0) - s) / row[i];
+    if (lane == 0) {
+      xc[i - c] = xi;
+      X[i * n + c] = xi;
+    }
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
@@ -274,7 +283,11 @@ int vjp_flow(const RefineProblem& p, const VjpIO& io, void* ws, size_t ws_bytes,
                      n0, b.vb, b.xs);
   hipLaunchKernelGGL(trmv_t_kernel, dim3(grid_for(n0, 4)), dim3(256), 0, s, b.L, n0, b.vb, b.u, p.perm, io.gxi,
                      io.gxi_f64);
-  hipLaunchKernelGGL(tri_inv_kernel, dim3(grid_for(n0, 64)), dim3(64), 0, s, b.L, n0, b.X);
+  const size_t xl = (size_t)n0 * sizeof(double);
+  GGP_CHECK_ARG(xl <= 160 * 1024, "generate_vjp: seed block too large for the LDS column inverse (n0 <= 20480)");
+  if (xl > 65536)
+    GGP_HIP(hipFuncSetAttribute((const void*)tri_inv_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)xl));
+  hipLaunchKernelGGL(tri_inv_kernel, dim3((unsigned)n0), dim3(64), xl, s, b.L, n0, b.X);
   hipLaunchKernelGGL(phi_kernel, dim3(grid_for(n0, 64)), dim3(64), 0, s, b.X, n0, b.u, b.xs, b.Tm);
   hipLaunchKernelGGL(gemm_tn_kernel, dim3(grid_for(n0, 64), grid_for(n0, 64)), dim3(256), 0, s, b.X, b.Tm, n0, b.S);
   GGP_LAUNCH_CHECK();

@@ -17,9 +17,11 @@
 // substitutions on it give w and z; the table gradient goes to an FP64 histogram in
 // LDS (ds_add_f64) flushed once per workgroup.  Points are taken in decreasing order
 // from an atomic ticket; a point starts its scatter once its count of unfinished
-// children reaches zero (acquire), and each scatter ends with a release decrement of
-// the neighbour's count, so the sweep is one launch and dependencies always lie on
-// lower tickets.
+// children reaches zero, so the sweep is one launch and dependencies always lie on lower
+// tickets.  vbar is only ever changed by device-scope atomics and read by a device-scope
+// atomic load, so no cache maintenance is needed: a child waits for its additions to
+// complete (s_waitcnt) before it decrements the counts, and a parent reads vbar only after
+// it has observed its count at zero.
 #pragma once
 #include "refine_impl.hpp"
 
@@ -51,7 +53,7 @@ struct AdjointArgs {
 };
 
 template <typename T, int KK, int P, int D, bool FAST>
-__global__ __launch_bounds__(256) void adjoint_kernel(AdjointArgs<T> a) {
+__global__ __launch_bounds__(256, 2) void adjoint_kernel(AdjointArgs<T> a) {
   static_assert(KK % P == 0, "KK must be a multiple of P");
   constexpr int R = KK / P;
   constexpr int S = P * R * (R + 1) / 2;
@@ -129,20 +131,25 @@ __global__ __launch_bounds__(256) void adjoint_kernel(AdjointArgs<T> a) {
     const bool valid = pt >= 0;
     const int64_t fine = a.n0 + pt;
 
-    T c[R][D];
     int32_t nid[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int i = P * r + lane;
-      int64_t id = -1;
-      if (valid) id = i < k ? (int64_t)a.nbrs[pt * k + i] : (i == k ? fine : -1);
-      nid[r] = (int32_t)id;
-#pragma unroll
-      for (int j = 0; j < D; ++j) c[r][j] = (id >= 0 && j < a.d) ? load_as<T>(a.pts, a.pts_f64, id * a.d + j) : T(0);
+      nid[r] = (valid && i < k) ? a.nbrs[pt * k + i] : -1;
     }
-    T cf[D];  // the fine point
+    // coordinates of the rows and of the fine point; loaded again (L2-resident) for the
+    // gradient phase instead of being kept live through the factorisation
+    auto load_rows = [&](T (&cc)[R][D], T (&ccf)[D]) {
 #pragma unroll
-    for (int j = 0; j < D; ++j) cf[j] = (valid && j < a.d) ? load_as<T>(a.pts, a.pts_f64, fine * a.d + j) : T(0);
+      for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int j = 0; j < D; ++j)
+          cc[r][j] = (nid[r] >= 0 && j < a.d) ? load_as<T>(a.pts, a.pts_f64, (int64_t)nid[r] * a.d + j) : T(0);
+#pragma unroll
+      for (int j = 0; j < D; ++j) ccf[j] = (valid && j < a.d) ? load_as<T>(a.pts, a.pts_f64, fine * a.d + j) : T(0);
+    };
+    T c[R][D], cf[D];
+    load_rows(c, cf);
 
     // joint covariance of the neighbours (rows < k) and the cross-covariance b
     T m[S];
@@ -239,7 +246,7 @@ __global__ __launch_bounds__(256) void adjoint_kernel(AdjointArgs<T> a) {
     bool done = !valid;
     for (uint32_t it = 0;; ++it) {
       if (!done) {
-        const uint32_t left = __hip_atomic_load(&a.children[fine], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t left = __hip_atomic_load(&a.children[fine], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const uint64_t bal = __ballot(left == 0);
         if (((bal >> gbase) & gmask) == gmask) {
           const double s = (double)__hip_atomic_load(&a.vbar[fine], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -250,32 +257,46 @@ __global__ __launch_bounds__(256) void adjoint_kernel(AdjointArgs<T> a) {
             else ((float*)a.gxi)[dst] = (float)gx;
           }
 #pragma unroll
-          for (int r = 0; r < R; ++r) {
-            if (P * r + lane < k) {
-              atomicAdd(&a.vbar[nid[r]], (T)(s * (double)w[r]));
-              __hip_atomic_fetch_sub(&a.children[nid[r]], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            }
-          }
+          for (int r = 0; r < R; ++r)
+            if (P * r + lane < k)
+              __hip_atomic_fetch_add(&a.vbar[nid[r]], (T)(s * (double)w[r]), __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+          // every addition has completed at its device-scope destination before any count drops
+          __atomic_signal_fence(__ATOMIC_SEQ_CST);
+          __builtin_amdgcn_s_waitcnt(0);
+          __atomic_signal_fence(__ATOMIC_SEQ_CST);
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+            if (P * r + lane < k)
+              __hip_atomic_fetch_sub(&a.children[nid[r]], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           if (s != 0.0) {
             const double sx = s * xi;
-            // entries of A (diagonal and below), then b, then c
-            static_for<0, KK>([&](auto jc) {
-              constexpr int j = decltype(jc)::value;
-              constexpr int js = j % P, jr = j / P;
-              if (j < k) {
-                T cj[D];
+            T c[R][D], cf[D];
+            load_rows(c, cf);
+            // entries of A (diagonal and below), then b, then c.  Unrolled over the row
+            // slot jr only; the column's lane js is a run-time loop (ds_bpermute broadcast),
+            // which keeps the register footprint of this phase small.
+            static_for<0, R>([&](auto jrc) {
+              constexpr int jr = decltype(jrc)::value;
+#pragma unroll 1
+              for (int js = 0; js < P; ++js) {
+                const int j = P * jr + js;
+                if (j < k) {
+                  const int src = gbase + js;
+                  T cj[D];
 #pragma unroll
-                for (int t = 0; t < D; ++t) cj[t] = bcast<P, js>(c[jr][t]);
-                const double wj = (double)bcast<P, js>(w[jr]);
-                const double zj = (double)bcast<P, js>(z[jr]);
+                  for (int t = 0; t < D; ++t) cj[t] = __shfl(c[jr][t], src, 64);
+                  const double wj = (double)__shfl(w[jr], src, 64);
+                  const double zj = (double)__shfl(z[jr], src, 64);
 #pragma unroll
-                for (int r = jr; r < R; ++r) {
-                  const int i = P * r + lane;
-                  if (i >= j && i < k) {
-                    const double wi = (double)w[r], zi = (double)z[r];
-                    const double g = (i == j) ? (-s * wi * zi + sx * wi * wi / (2.0 * sigma))
-                                              : (-s * (wi * zj + wj * zi) + sx * wi * wj / sigma);
-                    grad_entry(pair_dist2<T, D>(c[r], cj), g);
+                  for (int r = jr; r < R; ++r) {
+                    const int i = P * r + lane;
+                    if (i >= j && i < k) {
+                      const double wi = (double)w[r], zi = (double)z[r];
+                      const double g = (i == j) ? (-s * wi * zi + sx * wi * wi / (2.0 * sigma))
+                                                : (-s * (wi * zj + wj * zi) + sx * wi * wj / sigma);
+                      grad_entry(pair_dist2<T, D>(c[r], cj), g);
+                    }
                   }
                 }
               }

@@ -89,9 +89,11 @@ __device__ __forceinline__ int bcast_i32(int x) {
     return __builtin_amdgcn_update_dpp(0, x, q, 0xF, 0xF, false);
   } else if constexpr (P == 4) {
     return __builtin_amdgcn_update_dpp(0, x, S * 0x55, 0xF, 0xF, false);
-  } else {
-    static_assert(P == 8, "P must be 1, 2, 4 or 8");
+  } else if constexpr (P == 8) {
     return __builtin_amdgcn_ds_swizzle(x, 0x18 | (S << 5));
+  } else {
+    static_assert(P == 16, "P must be 1, 2, 4, 8 or 16");
+    return __builtin_amdgcn_ds_swizzle(x, 0x10 | (S << 5));
   }
 }
 template <int P, int S>
