@@ -31,7 +31,7 @@ _SIGS = {
     "ggp_query_neighbors_workspace": (_c_size, [_i64, _int, _int]),
     "ggp_query_preceding_neighbors": (_int, [_p, _p, _i64, _int, _int, _i64, _int, _p, _p, _c_size, _p]),
     "ggp_query_neighbors": (_int, [_p, _p, _i64, _int, _int, _p, _p, _i64, _int, _p, _p, _c_size, _p]),
-    "ggp_compute_depths_workspace": (_c_size, []),
+    "ggp_compute_depths_workspace": (_c_size, [_i64, _int]),
     "ggp_compute_depths": (_int, [_p, _i64, _int, _i64, _p, _p, _c_size, _p]),
     "ggp_compute_depths_status": (_int, [_p, _u32p, _p]),
     "ggp_order_by_depth_workspace": (_c_size, [_i64]),

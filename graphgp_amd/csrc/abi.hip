@@ -59,7 +59,7 @@ struct GraphBufs {
 size_t graph_sub_bytes(int64_t n, int d, int dtype, int64_t n0, int k) {
   size_t a = build_tree_workspace(n, d, dtype);
   size_t b = query_neighbors_workspace(n - n0, k, dtype);
-  size_t c = compute_depths_workspace();
+  size_t c = compute_depths_workspace(n - n0, k);
   size_t e = order_by_depth_workspace(n);
   return std::max(std::max(a, b), std::max(c, e));
 }
@@ -150,7 +150,7 @@ int ggp_query_neighbors(const void* points, const int32_t* split_dims, int64_t n
                          workspace, workspace_bytes, S(stream));
 }
 
-size_t ggp_compute_depths_workspace(void) { return compute_depths_workspace(); }
+size_t ggp_compute_depths_workspace(int64_t n_ref, int k) { return compute_depths_workspace(n_ref, k); }
 
 int ggp_compute_depths(const int32_t* neighbors, int64_t n_ref, int k, int64_t n0, int32_t* depths, void* workspace,
                        size_t workspace_bytes, void* stream) {

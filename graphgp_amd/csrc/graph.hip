@@ -5,11 +5,15 @@
 //
 // compute_depths: depth[i] = 1 + max depth[neighbours(i)] (depth 0 for the n0 seed
 // points).  The reference iterates that map to its fixed point with Jacobi sweeps (one
-// sweep per depth level).  Here each sweep is Gauss-Seidel in index order (in place,
-// agent-coherent loads and stores, no waiting), so a dependency is usually resolved in
-// the sweep that reaches it and a handful of sweeps suffice; the host stops at the first
-// sweep that changes nothing, which is the reference's convergence test.  The fixed
-// point of a graph in topological order is unique, so the result is the reference's.
+// sweep over all N points per depth level).  Here the longest-path depths come from
+// Kahn's algorithm by rounds: the transposed graph (each point's children) is built once
+// (count, scan, fill); round d then takes the frontier of points whose last unfinished
+// neighbour finished in round d-1, so every edge is touched a constant number of times
+// and a round costs one launch over its frontier.  A point leaves the frontier in the
+// round equal to its longest path from the seeds, which is the reference's fixed point
+// (unique for a graph in topological order).  When the caller's workspace cannot hold the
+// transpose, Gauss-Seidel sweeps in index order are used instead (in place,
+// agent-coherent loads and stores, no waiting) until a sweep changes nothing.
 //
 // order_by_depth: stable sort by depth (the radix sort of radix.hip), gathers, and the
 // neighbour remap through the inverse permutation.
@@ -52,6 +56,162 @@ __global__ __launch_bounds__(kDepThreads) void depths_sweep_kernel(const int32_t
   }
   if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(status, 1u);
   if (__any(changed) && (threadIdx.x & 63) == 0) atomicOr(status, 2u);
+}
+
+// ---- Kahn rounds.  ctr: [0] malformed flag, [1] final status, [2] head, [3] end, [4] tail,
+// [5] depth of the current frontier, [6] finished workgroups of the current round.
+constexpr int kCtrWords = 8;
+
+// Validate row r, count its non-seed neighbours, count children, seed the first frontier.
+__global__ void kahn_init_kernel(const int32_t* __restrict__ nbrs, int64_t n_ref, int k, int64_t n0,
+                                 uint32_t* __restrict__ remaining, uint32_t* __restrict__ cnt,
+                                 uint32_t* __restrict__ queue, int32_t* __restrict__ depths, uint32_t* ctr) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool bad = false;
+  if (r < n_ref) {
+    const int64_t i = n0 + r;
+    uint32_t rem = 0;
+    for (int j = 0; j < k; ++j) {
+      const int32_t nb = nbrs[r * k + j];
+      if (nb < 0 || (int64_t)nb >= i) {
+        bad = true;
+        break;
+      }
+      if (nb >= n0) ++rem;
+    }
+    if (!bad) {
+      for (int j = 0; j < k; ++j) {
+        const int32_t nb = nbrs[r * k + j];
+        if (nb >= n0) atomicAdd(&cnt[nb - n0], 1u);
+      }
+      remaining[r] = rem;
+      if (rem == 0) {
+        depths[i] = 1;
+        queue[atomicAdd(&ctr[4], 1u)] = (uint32_t)r;
+      }
+    }
+  }
+  if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(&ctr[0], 1u);
+}
+
+// children of refined point j at child[rowptr[j] .. rowptr[j+1])
+__global__ void kahn_fill_kernel(const int32_t* __restrict__ nbrs, int64_t n_ref, int k, int64_t n0,
+                                 const int64_t* __restrict__ rowptr, uint32_t* __restrict__ cur,
+                                 uint32_t* __restrict__ child) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_ref * k) return;
+  const int32_t nb = nbrs[e];
+  if (nb < n0) return;
+  const int64_t j = nb - n0;
+  child[rowptr[j] + atomicAdd(&cur[j], 1u)] = (uint32_t)(e / k);
+}
+
+__global__ void kahn_start_kernel(uint32_t* ctr) {
+  ctr[2] = 0;
+  ctr[3] = ctr[4];
+  ctr[5] = 1;
+  ctr[6] = 0;
+}
+
+// One round: every child of a frontier point loses one unfinished neighbour; a child that
+// reaches zero gets depth d + 1 and joins the next frontier.  The last workgroup to finish
+// moves [head, end) to the points queued in this round.
+__global__ __launch_bounds__(256) void kahn_round_kernel(const int64_t* __restrict__ rowptr,
+                                                         const uint32_t* __restrict__ child, int64_t n0,
+                                                         uint32_t* __restrict__ remaining, uint32_t* queue,
+                                                         int32_t* __restrict__ depths, uint32_t* ctr) {
+  const uint32_t head = __hip_atomic_load(&ctr[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t end = __hip_atomic_load(&ctr[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int32_t d = (int32_t)__hip_atomic_load(&ctr[5], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t q = head + blockIdx.x * blockDim.x + threadIdx.x; q < end; q += stride) {
+    const uint32_t j = queue[q];
+    const int64_t p1 = rowptr[j + 1];
+    for (int64_t p = rowptr[j]; p < p1; ++p) {
+      const uint32_t i = child[p];
+      if (atomicSub(&remaining[i], 1u) == 1u) {
+        depths[n0 + i] = d + 1;
+        queue[atomicAdd(&ctr[4], 1u)] = i;
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&ctr[6], 1u) == gridDim.x - 1) {
+      const uint32_t tail = __hip_atomic_load(&ctr[4], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&ctr[2], end, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&ctr[3], tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&ctr[5], (uint32_t)(d + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&ctr[6], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+// ---- exclusive scan of n uint32 counts into n + 1 int64 offsets (three phases)
+constexpr int kScanTile = 2048;  // 256 threads x 8
+
+__global__ __launch_bounds__(256) void scan_tile_sum_kernel(const uint32_t* __restrict__ in, int64_t n,
+                                                            int64_t* __restrict__ sums) {
+  __shared__ int64_t red[4];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  int64_t v = 0;
+  for (int t = threadIdx.x; t < kScanTile; t += 256)
+    if (base + t < n) v += in[base + t];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) sums[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// in-place exclusive scan of the tile sums by one workgroup
+__global__ __launch_bounds__(1024) void scan_sums_kernel(int64_t* sums, int64_t m) {
+  __shared__ int64_t buf[1024];
+  int64_t carry = 0;
+  for (int64_t base = 0; base < m; base += 1024) {
+    const int64_t i = base + threadIdx.x;
+    const int64_t v = i < m ? sums[i] : 0;
+    buf[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+      const int64_t t = threadIdx.x >= o ? buf[threadIdx.x - o] : 0;
+      __syncthreads();
+      buf[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < m) sums[i] = carry + buf[threadIdx.x] - v;
+    const int64_t tot = buf[1023];
+    __syncthreads();
+    carry += tot;
+  }
+}
+
+__global__ __launch_bounds__(256) void scan_tile_kernel(const uint32_t* __restrict__ in, int64_t n,
+                                                        const int64_t* __restrict__ sums, int64_t* __restrict__ out) {
+  __shared__ int64_t buf[256];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * 8;
+  uint32_t v[8];
+  int64_t loc = 0;
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    v[t] = base + t < n ? in[base + t] : 0u;
+    loc += v[t];
+  }
+  buf[threadIdx.x] = loc;
+  __syncthreads();
+  for (int o = 1; o < 256; o <<= 1) {
+    const int64_t t = threadIdx.x >= o ? buf[threadIdx.x - o] : 0;
+    __syncthreads();
+    buf[threadIdx.x] += t;
+    __syncthreads();
+  }
+  int64_t run = sums[blockIdx.x] + buf[threadIdx.x] - loc;
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    if (base + t < n) out[base + t] = run;
+    run += v[t];
+  }
+  if (base < n && base + 8 >= n) out[n] = run;
 }
 
 __global__ void depth_init_kernel(int32_t* depths, int64_t n) {
@@ -137,25 +297,28 @@ __global__ void check_graph_kernel(const int32_t* __restrict__ nbrs, int64_t n_r
 
 }  // namespace
 
-size_t compute_depths_workspace() { return 512; }
+namespace {
 
-// Sweeps until nothing changes (synchronises once per sweep).  The number of sweeps is
-// at most max depth + 1 and in practice a handful: most neighbours lie far earlier in
-// the order and are final before the sweep reaches their dependents.
-int compute_depths(const int32_t* nbrs, int64_t n_ref, int k, int64_t n0, int32_t* depths, void* ws,
-                   size_t ws_bytes, hipStream_t s) {
-  GGP_CHECK_ARG(n0 >= 0 && n_ref >= 0 && k >= 1, "compute_depths: bad sizes");
-  Workspace w(ws, ws_bytes);
-  uint32_t* ctr = w.take<uint32_t>(2);
-  if (!w.ok()) {
-    set_error("compute_depths: workspace too small");
-    return kWorkspaceTooSmall;
-  }
-  const int64_t n = n0 + n_ref;
-  GGP_TRY(fill_u32(ctr, 0u, 2, s));
-  hipLaunchKernelGGL(depth_init_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, depths, n);
-  GGP_LAUNCH_CHECK();
-  if (n_ref == 0) return kOk;
+struct KahnWs {
+  uint32_t* ctr;
+  uint32_t *remaining, *cnt, *queue, *child;
+  int64_t *rowptr, *sums;
+};
+
+bool carve_kahn(Workspace& w, int64_t n_ref, int k, KahnWs& b) {
+  b.ctr = w.take<uint32_t>(kCtrWords);
+  b.remaining = w.take<uint32_t>(n_ref + 1);
+  b.cnt = w.take<uint32_t>(n_ref + 1);
+  b.queue = w.take<uint32_t>(n_ref + 1);
+  b.rowptr = w.take<int64_t>(n_ref + 1);
+  b.sums = w.take<int64_t>((n_ref + kScanTile - 1) / kScanTile + 1);
+  b.child = w.take<uint32_t>(n_ref * k + 1);
+  return w.ok();
+}
+
+// Gauss-Seidel sweeps until nothing changes (one host synchronisation per sweep).
+int depths_by_sweeps(const int32_t* nbrs, int64_t n_ref, int k, int64_t n0, int32_t* depths, uint32_t* ctr,
+                     hipStream_t s) {
   uint32_t status = 0;
   for (int64_t sweep = 0; sweep <= n_ref + 1; ++sweep) {
     GGP_TRY(fill_u32(ctr, 0u, 1, s));
@@ -164,11 +327,82 @@ int compute_depths(const int32_t* nbrs, int64_t n_ref, int k, int64_t n0, int32_
     GGP_LAUNCH_CHECK();
     GGP_HIP(hipMemcpyAsync(&status, ctr, 4, hipMemcpyDeviceToHost, s));
     GGP_HIP(hipStreamSynchronize(s));
-    if (status & 1u) break;      // malformed graph: report through the status word
-    if (!(status & 2u)) break;   // fixed point
+    if (status & 1u) break;     // malformed graph: report through the status word
+    if (!(status & 2u)) break;  // fixed point
   }
   uint32_t final_status = (status & 1u) ? 1u : 0u;
   GGP_HIP(hipMemcpyAsync(ctr + 1, &final_status, 4, hipMemcpyHostToDevice, s));
+  GGP_HIP(hipStreamSynchronize(s));
+  return kOk;
+}
+
+}  // namespace
+
+size_t compute_depths_workspace(int64_t n_ref, int k) {
+  Workspace w(nullptr, 0);
+  KahnWs b;
+  carve_kahn(w, n_ref, k, b);
+  return w.used + 256;
+}
+
+// Kahn rounds when the workspace holds the transpose (compute_depths_workspace), else
+// sweeps (the workspace then needs only its first 32 bytes).  Synchronises with the host:
+// once after validation and once per 32 rounds.
+int compute_depths(const int32_t* nbrs, int64_t n_ref, int k, int64_t n0, int32_t* depths, void* ws,
+                   size_t ws_bytes, hipStream_t s) {
+  GGP_CHECK_ARG(n0 >= 0 && n_ref >= 0 && k >= 1, "compute_depths: bad sizes");
+  GGP_CHECK_ARG(n_ref < (int64_t(1) << 31), "compute_depths: at most 2^31 refined points");
+  Workspace w(ws, ws_bytes);
+  KahnWs b;
+  const bool kahn = carve_kahn(w, n_ref, k, b);
+  if (b.ctr == nullptr) {
+    set_error("compute_depths: workspace too small");
+    return kWorkspaceTooSmall;
+  }
+  const int64_t n = n0 + n_ref;
+  GGP_TRY(fill_u32(b.ctr, 0u, kCtrWords, s));
+  hipLaunchKernelGGL(depth_init_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, depths, n);
+  GGP_LAUNCH_CHECK();
+  if (n_ref == 0) return kOk;
+  if (!kahn) return depths_by_sweeps(nbrs, n_ref, k, n0, depths, b.ctr, s);
+
+  GGP_TRY(fill_u32(b.cnt, 0u, n_ref, s));
+  hipLaunchKernelGGL(kahn_init_kernel, dim3(grid_for(n_ref, 256)), dim3(256), 0, s, nbrs, n_ref, k, n0, b.remaining,
+                     b.cnt, b.queue, depths, b.ctr);
+  GGP_LAUNCH_CHECK();
+  uint32_t bad = 0;
+  GGP_HIP(hipMemcpyAsync(&bad, b.ctr, 4, hipMemcpyDeviceToHost, s));
+  GGP_HIP(hipStreamSynchronize(s));
+  if (bad) {
+    const uint32_t one = 1;
+    GGP_HIP(hipMemcpyAsync(b.ctr + 1, &one, 4, hipMemcpyHostToDevice, s));
+    GGP_HIP(hipStreamSynchronize(s));
+    return kOk;
+  }
+  const int64_t tiles = (n_ref + kScanTile - 1) / kScanTile;
+  hipLaunchKernelGGL(scan_tile_sum_kernel, dim3((unsigned)tiles), dim3(256), 0, s, b.cnt, n_ref, b.sums);
+  hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(1024), 0, s, b.sums, tiles);
+  hipLaunchKernelGGL(scan_tile_kernel, dim3((unsigned)tiles), dim3(256), 0, s, b.cnt, n_ref, b.sums, b.rowptr);
+  GGP_LAUNCH_CHECK();
+  GGP_TRY(fill_u32(b.cnt, 0u, n_ref, s));
+  hipLaunchKernelGGL(kahn_fill_kernel, dim3(grid_for(n_ref * k, 256)), dim3(256), 0, s, nbrs, n_ref, k, n0,
+                     b.rowptr, b.cnt, b.child);
+  hipLaunchKernelGGL(kahn_start_kernel, dim3(1), dim3(1), 0, s, b.ctr);
+  GGP_LAUNCH_CHECK();
+  const unsigned grid = 512;
+  uint32_t hdr[3] = {0, 0, 0};
+  for (int64_t round = 0; round <= n_ref + 1; round += 32) {
+    for (int q = 0; q < 32; ++q)
+      hipLaunchKernelGGL(kahn_round_kernel, dim3(grid), dim3(256), 0, s, b.rowptr, b.child, n0, b.remaining,
+                         b.queue, depths, b.ctr);
+    GGP_LAUNCH_CHECK();
+    GGP_HIP(hipMemcpyAsync(hdr, b.ctr + 2, 12, hipMemcpyDeviceToHost, s));
+    GGP_HIP(hipStreamSynchronize(s));
+    if (hdr[0] == hdr[1]) break;  // empty frontier
+  }
+  // every refined point must have left a frontier (always true for a graph in topological order)
+  const uint32_t final_status = hdr[2] == (uint32_t)n_ref ? 0u : 1u;
+  GGP_HIP(hipMemcpyAsync(b.ctr + 1, &final_status, 4, hipMemcpyHostToDevice, s));
   GGP_HIP(hipStreamSynchronize(s));
   return kOk;
 }

@@ -15,7 +15,7 @@ int query_neighbors(const void* pts, const int32_t* split, int64_t n, int d, int
                     const int32_t* midx, int64_t q0, int64_t nq, int k, int32_t* out, void* ws, size_t ws_bytes,
                     hipStream_t s);
 
-size_t compute_depths_workspace();
+size_t compute_depths_workspace(int64_t n_ref, int k);
 int compute_depths(const int32_t* nbrs, int64_t n_ref, int k, int64_t n0, int32_t* depths, void* ws,
                    size_t ws_bytes, hipStream_t s);
 size_t order_by_depth_workspace(int64_t n);

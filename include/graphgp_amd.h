@@ -60,9 +60,11 @@ int ggp_query_neighbors(const void* points, const int32_t* split_dims, int64_t n
 
 /* ---- depths: replaces graphgp_cuda.compute_depths_parallel(neighbors, n0=),
  * graphgp/graph.py:106.  depths (n0 + n_ref).  A malformed graph (a neighbour that does
- * not precede its point) is reported by ggp_compute_depths_status (syncs): bit 0 =
- * non-preceding neighbour, bit 1 = wait limit reached. */
-size_t ggp_compute_depths_workspace(void);
+ * not precede its point) is reported by ggp_compute_depths_status (syncs): status 1.
+ * With the workspace of ggp_compute_depths_workspace (holds the transposed graph) the
+ * depths come from Kahn rounds; a workspace of only 64 bytes selects in-place sweeps
+ * (slower, no transpose) for graphs whose transpose would not fit in device memory. */
+size_t ggp_compute_depths_workspace(int64_t n_ref, int k);
 int ggp_compute_depths(const int32_t* neighbors, int64_t n_ref, int k, int64_t n0, int32_t* depths, void* workspace,
                        size_t workspace_bytes, void* stream);
 int ggp_compute_depths_status(void* workspace, uint32_t* host_status, void* stream);

@@ -107,10 +107,27 @@ def test_query_n0_less_than_k_raises():
 
 
 # ---------------------------------------------------------------- graph
-def test_compute_depths_kat():
+@pytest.mark.parametrize("method", ["rounds", "sweeps"])
+def test_compute_depths_kat(method):
     # tests/test_graph.py:40-52
     nb = np.array([[0, 1, 2], [0, 1, 2], [1, 3, 4], [2, 4, 5]])
-    assert np.array_equal(cpu(gp.compute_depths(nb, n0=3)), [0, 0, 0, 1, 1, 2, 3])
+    assert np.array_equal(cpu(gp.compute_depths(nb, n0=3, method=method)), [0, 0, 0, 1, 1, 2, 3])
+
+
+@pytest.mark.parametrize("method", ["rounds", "sweeps"])
+def test_compute_depths_random_vs_oracle(method):
+    # random graphs in topological order with repeated neighbours, and a 3000-long chain
+    # (3000 rounds: more than one batch of rounds between host checks)
+    rng = np.random.default_rng(5)
+    for n0, n_ref, k in [(5, 2000, 4), (50, 30000, 8), (1, 3000, 1)]:
+        hi = n0 + np.arange(n_ref)[:, None]
+        if k == 1:
+            nb = (hi - 1).astype(np.int32)
+        else:
+            nb = np.minimum(rng.integers(0, hi, (n_ref, k)), hi - 1 - rng.integers(0, 3, (n_ref, k)))
+            nb = np.maximum(nb, 0).astype(np.int32)
+        d = cpu(gp.compute_depths(nb, n0=n0, method=method))
+        assert np.array_equal(d, O.compute_depths(nb, n0=n0)), (n0, n_ref, k)
 
 
 def test_order_by_depth_kat():
@@ -124,9 +141,12 @@ def test_order_by_depth_kat():
     assert np.array_equal(cpu(d), [0, 0, 1, 1, 2, 3])
 
 
-def test_compute_depths_rejects_malformed():
+@pytest.mark.parametrize("method", ["rounds", "sweeps"])
+def test_compute_depths_rejects_malformed(method):
     with pytest.raises(ValueError):
-        gp.compute_depths(np.array([[0, 4], [1, 2]]), n0=3)
+        gp.compute_depths(np.array([[0, 4], [1, 2]]), n0=3, method=method)
+    with pytest.raises(ValueError):
+        gp.compute_depths(np.array([[0, 1], [-1, 2]]), n0=3, method=method)
 
 
 def test_build_graph_reference_kat():
