@@ -28,7 +28,7 @@ _SIGS = {
     "ggp_abi_version": (_int, []),
     "ggp_build_tree_workspace": (_c_size, [_i64, _int, _int]),
     "ggp_build_tree": (_int, [_p, _i64, _int, _int, _p, _p, _p, _p, _c_size, _p]),
-    "ggp_query_neighbors_workspace": (_c_size, [_i64, _int, _int]),
+    "ggp_query_neighbors_workspace": (_c_size, [_i64, _int, _i64, _int, _int]),
     "ggp_query_preceding_neighbors": (_int, [_p, _p, _i64, _int, _int, _i64, _int, _p, _p, _c_size, _p]),
     "ggp_query_neighbors": (_int, [_p, _p, _i64, _int, _int, _p, _p, _i64, _int, _p, _p, _c_size, _p]),
     "ggp_compute_depths_workspace": (_c_size, [_i64, _int]),

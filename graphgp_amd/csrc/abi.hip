@@ -58,7 +58,7 @@ struct GraphBufs {
 
 size_t graph_sub_bytes(int64_t n, int d, int dtype, int64_t n0, int k) {
   size_t a = build_tree_workspace(n, d, dtype);
-  size_t b = query_neighbors_workspace(n - n0, k, dtype);
+  size_t b = query_neighbors_workspace(n, d, n - n0, k, dtype);
   size_t c = compute_depths_workspace(n - n0, k);
   size_t e = order_by_depth_workspace(n);
   return std::max(std::max(a, b), std::max(c, e));
@@ -124,8 +124,8 @@ int ggp_build_tree(const void* points, int64_t n, int d, int dtype, void* points
   return build_tree(points, n, d, dtype, points_out, split_dims, indices, workspace, workspace_bytes, S(stream));
 }
 
-size_t ggp_query_neighbors_workspace(int64_t n_queries, int k, int dtype) {
-  return query_neighbors_workspace(n_queries, k, dtype);
+size_t ggp_query_neighbors_workspace(int64_t n, int d, int64_t n_queries, int k, int dtype) {
+  return query_neighbors_workspace(n, d, n_queries, k, dtype);
 }
 
 int ggp_query_preceding_neighbors(const void* points, const int32_t* split_dims, int64_t n, int d, int dtype,

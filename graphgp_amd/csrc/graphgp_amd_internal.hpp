@@ -10,7 +10,7 @@ size_t build_tree_workspace(int64_t n, int d, int dtype);
 int build_tree(const void* pts, int64_t n, int d, int dtype, void* pts_out, int32_t* split_dims, int32_t* indices,
                void* ws, size_t ws_bytes, hipStream_t s);
 
-size_t query_neighbors_workspace(int64_t nq, int k, int dtype);
+size_t query_neighbors_workspace(int64_t n, int d, int64_t nq, int k, int dtype);
 int query_neighbors(const void* pts, const int32_t* split, int64_t n, int d, int dtype, const int32_t* qidx,
                     const int32_t* midx, int64_t q0, int64_t nq, int k, int32_t* out, void* ws, size_t ws_bytes,
                     hipStream_t s);

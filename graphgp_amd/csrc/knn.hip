@@ -11,6 +11,15 @@
 // rounded arithmetic as the reference (this file is built with -ffp-contract=off), so
 // indices match bit for bit, ties included.
 //
+// Bounding-box pruning: each subtree's axis-aligned bounding box is built bottom-up, and
+// a child is skipped when its box distance^2 is strictly greater than r^2.  Every point
+// in such a subtree has distance^2 >= box distance^2 > r^2 (the box distance is summed in
+// the same order, with the same rounding, and each term is a monotone lower bound of the
+// point's term), so the reference would visit those points without inserting any of them,
+// and r^2 never grows: the candidate list evolves exactly as in the reference, ties
+// included.  The skip only removes work: near-empty visits around dense clusters, where
+// split-plane distances alone cannot prune, dominate the reference's worst queries.
+//
 // MI355X mapping: the k slots live in VGPRs (k <= 64, compile-time slot count KMAX with
 // slots >= k pinned at -inf so they never hold the maximum); the final order is a
 // register bitonic network.  Larger k uses slots in a caller workspace.
@@ -47,6 +56,58 @@ __device__ inline T dist2(const Query<T, D>& q, const T* __restrict__ p, int d) 
   return s;
 }
 
+// box of node c: lo at box[c*2d + j], hi at box[c*2d + d + j]; distance^2 from q to the
+// box, accumulated in the order of dist2.
+template <typename T, int D>
+__device__ inline T box_dist2(const Query<T, D>& q, const T* __restrict__ b, int d) {
+  T s = T(0);
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    if (j < d) {
+      const T lo = b[j] - q.c[j], hi = q.c[j] - b[d + j];
+      const T g = lo > T(0) ? lo : (hi > T(0) ? hi : T(0));
+      s = j == 0 ? g * g : s + g * g;
+    }
+  }
+  return s;
+}
+
+template <typename T>
+__device__ inline T box_dist2_rt(const T* __restrict__ q, const T* __restrict__ b, int d) {
+  T s = T(0);
+  for (int j = 0; j < d; ++j) {
+    const T lo = b[j] - q[j], hi = q[j] - b[d + j];
+    const T g = lo > T(0) ? lo : (hi > T(0) ? hi : T(0));
+    s = j == 0 ? g * g : s + g * g;
+  }
+  return s;
+}
+
+// boxes of the nodes of one tree level, from their own point and their children's boxes
+template <typename T>
+__global__ void box_level_kernel(const T* __restrict__ pts, int64_t n, int d, int64_t first, int64_t count,
+                                 T* __restrict__ box) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  const int64_t c = first + t;
+  if (c >= n) return;
+  const int64_t l = tree_left(c), r = tree_right(c);
+  T* b = box + c * 2 * d;
+  for (int j = 0; j < d; ++j) {
+    T lo = pts[c * d + j], hi = lo;
+    if (l < n) {
+      lo = fmin(lo, box[l * 2 * d + j]);
+      hi = fmax(hi, box[l * 2 * d + d + j]);
+    }
+    if (r < n) {
+      lo = fmin(lo, box[r * 2 * d + j]);
+      hi = fmax(hi, box[r * 2 * d + d + j]);
+    }
+    b[j] = lo;
+    b[d + j] = hi;
+  }
+}
+
 template <typename T>
 __device__ inline bool pair_greater(T da, int32_t ia, T db, int32_t ib) {
   return da > db || (da == db && ia > ib);
@@ -56,7 +117,7 @@ template <typename T, int D, int KMAX>
 __global__ __launch_bounds__(256) void knn_kernel(const T* __restrict__ pts, const int32_t* __restrict__ split,
                                                   int d, const int32_t* __restrict__ qidx,
                                                   const int32_t* __restrict__ midx, int64_t q0, int64_t nq, int k,
-                                                  int32_t* __restrict__ out, int64_t n) {
+                                                  int32_t* __restrict__ out, int64_t n, const T* __restrict__ box) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nq) return;
   const int64_t qi = qidx ? (int64_t)qidx[t] : q0 + t;
@@ -99,8 +160,9 @@ __global__ __launch_bounds__(256) void knn_kernel(const T* __restrict__ pts, con
     const int64_t farc = sdist < T(0) ? r : l;
     const bool far_in = sdist * sdist <= r2;
     int64_t next;
-    if (prev == nearc || (prev == parent && nearc >= max_index))
-      next = (farc < max_index && far_in) ? farc : parent;
+    if (prev == nearc ||
+        (prev == parent && (nearc >= max_index || box_dist2<T, D>(q, box + nearc * 2 * d, d) > r2)))
+      next = (farc < max_index && far_in && !(box_dist2<T, D>(q, box + farc * 2 * d, d) > r2)) ? farc : parent;
     else
       next = prev == parent ? nearc : parent;
     prev = cur;
@@ -143,7 +205,8 @@ __global__ __launch_bounds__(256) void knn_generic_kernel(const T* __restrict__ 
                                                           const int32_t* __restrict__ qidx,
                                                           const int32_t* __restrict__ midx, int64_t q0, int64_t nq,
                                                           int k, int32_t* __restrict__ out, T* __restrict__ wsd,
-                                                          int32_t* __restrict__ wsi, int64_t n) {
+                                                          int32_t* __restrict__ wsi, int64_t n,
+                                                          const T* __restrict__ box) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nq) return;
   const int64_t qi = qidx ? (int64_t)qidx[t] : q0 + t;
@@ -179,8 +242,8 @@ __global__ __launch_bounds__(256) void knn_generic_kernel(const T* __restrict__ 
     const int64_t farc = sdist < T(0) ? r : l;
     const bool far_in = sdist * sdist <= r2;
     int64_t next;
-    if (prev == nearc || (prev == parent && nearc >= max_index))
-      next = (farc < max_index && far_in) ? farc : parent;
+    if (prev == nearc || (prev == parent && (nearc >= max_index || box_dist2_rt<T>(q, box + nearc * 2 * d, d) > r2)))
+      next = (farc < max_index && far_in && !(box_dist2_rt<T>(q, box + farc * 2 * d, d) > r2)) ? farc : parent;
     else
       next = prev == parent ? nearc : parent;
     prev = cur;
@@ -205,18 +268,18 @@ __global__ __launch_bounds__(256) void knn_generic_kernel(const T* __restrict__ 
 
 template <typename T, int D, int KMAX>
 void launch_knn(const T* pts, const int32_t* split, int d, const int32_t* qidx, const int32_t* midx, int64_t q0,
-                int64_t nq, int k, int32_t* out, int64_t n, hipStream_t s) {
+                int64_t nq, int k, int32_t* out, int64_t n, const T* box, hipStream_t s) {
   hipLaunchKernelGGL((knn_kernel<T, D, KMAX>), dim3(grid_for(nq, 256)), dim3(256), 0, s, pts, split, d, qidx, midx,
-                     q0, nq, k, out, n);
+                     q0, nq, k, out, n, box);
 }
 
 template <typename T, int D>
 void dispatch_k(const T* pts, const int32_t* split, int d, const int32_t* qidx, const int32_t* midx, int64_t q0,
-                int64_t nq, int k, int32_t* out, int64_t n, hipStream_t s) {
-  if (k <= 8) launch_knn<T, D, 8>(pts, split, d, qidx, midx, q0, nq, k, out, n, s);
-  else if (k <= 16) launch_knn<T, D, 16>(pts, split, d, qidx, midx, q0, nq, k, out, n, s);
-  else if (k <= 32) launch_knn<T, D, 32>(pts, split, d, qidx, midx, q0, nq, k, out, n, s);
-  else launch_knn<T, D, 64>(pts, split, d, qidx, midx, q0, nq, k, out, n, s);
+                int64_t nq, int k, int32_t* out, int64_t n, const T* box, hipStream_t s) {
+  if (k <= 8) launch_knn<T, D, 8>(pts, split, d, qidx, midx, q0, nq, k, out, n, box, s);
+  else if (k <= 16) launch_knn<T, D, 16>(pts, split, d, qidx, midx, q0, nq, k, out, n, box, s);
+  else if (k <= 32) launch_knn<T, D, 32>(pts, split, d, qidx, midx, q0, nq, k, out, n, box, s);
+  else launch_knn<T, D, 64>(pts, split, d, qidx, midx, q0, nq, k, out, n, box, s);
 }
 
 template <typename T>
@@ -226,24 +289,37 @@ int knn_impl(const T* pts, const int32_t* split, int64_t n, int d, const int32_t
   GGP_CHECK_ARG(k >= 1, "query_neighbors: k >= 1");
   GGP_CHECK_ARG(n < (int64_t(1) << 31), "query_neighbors: N must be below 2^31");
   if (nq <= 0) return kOk;
+  Workspace w(ws, ws_bytes);
+  T* box = w.take<T>((size_t)n * 2 * d);
+  T* wsd = nullptr;
+  int32_t* wsi = nullptr;
   if (k > 64) {
-    Workspace w(ws, ws_bytes);
-    T* wsd = w.take<T>((size_t)nq * k);
-    int32_t* wsi = w.take<int32_t>((size_t)nq * k);
-    if (!w.ok()) {
-      set_error("query_neighbors: workspace too small for k > 64");
-      return kWorkspaceTooSmall;
-    }
+    wsd = w.take<T>((size_t)nq * k);
+    wsi = w.take<int32_t>((size_t)nq * k);
+  }
+  if (!w.ok()) {
+    set_error("query_neighbors: workspace too small");
+    return kWorkspaceTooSmall;
+  }
+  // subtree boxes, deepest level first
+  const int top = tree_level(n - 1);
+  for (int lv = top; lv >= 0; --lv) {
+    const int64_t first = (int64_t(1) << lv) - 1;
+    const int64_t count = std::min<int64_t>(int64_t(1) << lv, n - first);
+    hipLaunchKernelGGL(box_level_kernel<T>, dim3(grid_for(count, 256)), dim3(256), 0, s, pts, n, d, first, count, box);
+  }
+  GGP_LAUNCH_CHECK();
+  if (k > 64) {
     hipLaunchKernelGGL(knn_generic_kernel<T>, dim3(grid_for(nq, 256)), dim3(256), 0, s, pts, split, d, qidx, midx,
-                       q0, nq, k, out, wsd, wsi, n);
+                       q0, nq, k, out, wsd, wsi, n, (const T*)box);
   } else if (d == 1) {
-    dispatch_k<T, 1>(pts, split, d, qidx, midx, q0, nq, k, out, n, s);
+    dispatch_k<T, 1>(pts, split, d, qidx, midx, q0, nq, k, out, n, box, s);
   } else if (d == 2) {
-    dispatch_k<T, 2>(pts, split, d, qidx, midx, q0, nq, k, out, n, s);
+    dispatch_k<T, 2>(pts, split, d, qidx, midx, q0, nq, k, out, n, box, s);
   } else if (d == 3) {
-    dispatch_k<T, 3>(pts, split, d, qidx, midx, q0, nq, k, out, n, s);
+    dispatch_k<T, 3>(pts, split, d, qidx, midx, q0, nq, k, out, n, box, s);
   } else {
-    dispatch_k<T, 8>(pts, split, d, qidx, midx, q0, nq, k, out, n, s);
+    dispatch_k<T, 8>(pts, split, d, qidx, midx, q0, nq, k, out, n, box, s);
   }
   GGP_LAUNCH_CHECK();
   return kOk;
@@ -251,11 +327,13 @@ int knn_impl(const T* pts, const int32_t* split, int64_t n, int d, const int32_t
 
 }  // namespace
 
-size_t query_neighbors_workspace(int64_t nq, int k, int dtype) {
-  if (k <= 64) return 0;
+size_t query_neighbors_workspace(int64_t n, int d, int64_t nq, int k, int dtype) {
   Workspace w(nullptr, 0);
-  w.take<char>((size_t)nq * k * dtype_size(dtype));
-  w.take<int32_t>((size_t)nq * k);
+  w.take<char>((size_t)n * 2 * d * dtype_size(dtype));
+  if (k > 64) {
+    w.take<char>((size_t)nq * k * dtype_size(dtype));
+    w.take<int32_t>((size_t)nq * k);
+  }
   return w.used + 256;
 }
 

@@ -44,7 +44,7 @@ def query_preceding_neighbors(points, split_dims, *, n0: int, k: int, cuda: bool
     L = _lib.lib()
     dt = _lib.dtype_code(p)
     out = torch.empty((n - n0, k), dtype=torch.int32, device=p.device)
-    ws = _lib.workspace(L.ggp_query_neighbors_workspace(n - n0, k, dt))
+    ws = _lib.workspace(L.ggp_query_neighbors_workspace(n, d, n - n0, k, dt))
     _lib.check(L.ggp_query_preceding_neighbors(_lib.ptr(p), _lib.ptr(s), n, d, dt, n0, k, _lib.ptr(out),
                                                _lib.ptr(ws), ws.numel(), _lib.stream()), "query_preceding_neighbors")
     return out
@@ -60,7 +60,7 @@ def query_neighbors(points, split_dims, query_indices, max_indices, *, k: int, c
     L = _lib.lib()
     dt = _lib.dtype_code(p)
     out = torch.empty((q.numel(), k), dtype=torch.int32, device=p.device)
-    ws = _lib.workspace(L.ggp_query_neighbors_workspace(q.numel(), k, dt))
+    ws = _lib.workspace(L.ggp_query_neighbors_workspace(n, d, q.numel(), k, dt))
     _lib.check(L.ggp_query_neighbors(_lib.ptr(p), _lib.ptr(s), n, d, dt, _lib.ptr(q), _lib.ptr(m), q.numel(), k,
                                      _lib.ptr(out), _lib.ptr(ws), ws.numel(), _lib.stream()), "query_neighbors")
     return out

@@ -47,7 +47,7 @@ int ggp_build_tree(const void* points, int64_t n, int d, int dtype, void* points
 
 /* ---- preceding kNN: replaces graphgp_cuda.query_preceding_neighbors(points, split_dims,
  * n0=, k=), graphgp/tree.py:61.  neighbors (n - n0, k). */
-size_t ggp_query_neighbors_workspace(int64_t n_queries, int k, int dtype);
+size_t ggp_query_neighbors_workspace(int64_t n, int d, int64_t n_queries, int k, int dtype);
 int ggp_query_preceding_neighbors(const void* points, const int32_t* split_dims, int64_t n, int d, int dtype,
                                   int64_t n0, int k, int32_t* neighbors, void* workspace, size_t workspace_bytes,
                                   void* stream);
