@@ -74,6 +74,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--sample", type=int, default=2000, help="points checked against the oracle")
+    ap.add_argument("--serve-requests", type=int, default=200, help="requests of the serving run (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); every box in this setup has one GPU",
@@ -217,6 +218,13 @@ def main():
         }
         log(f"cpu baseline: {m / dt:.0f} pts/s on {cores} threads")
 
+    serve = None
+    if args.serve_requests > 0:
+        from graphgp_amd.serving import serve_benchmark
+
+        serve = serve_benchmark(args.serve_requests)
+        log(f"serving: {serve}")
+
     line = {
         "metric": "refined points/sec at N=16M,k=30 (FP32 vs plain FP64); p50/p99 serve latency",
         "value": round(r32["pts_per_s"], 1),
@@ -246,6 +254,7 @@ def main():
         "build_graph_ms": round(build_s * 1e3, 1),
         "roofline": roofline,
         "cpu_baseline": cpu_baseline,
+        "serve": serve,
     }
     print(json.dumps(line), flush=True)
 

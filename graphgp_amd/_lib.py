@@ -62,6 +62,7 @@ _SIGS = {
     "ggp_cov_matrix": (_int, [_p, _i64, _p, _i64, _int, _int, _p, _p, _int, _p, _p]),
     "ggp_profile_events": (_int, [_p, _p]),
     "ggp_refine_counters": (_int, [_p, _u32p, _p]),
+    "ggp_set_cu_budget": (_int, [_int]),
 }
 
 _lib = None

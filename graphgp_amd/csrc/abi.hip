@@ -13,10 +13,12 @@ extern thread_local hipEvent_t g_prof_start, g_prof_stop;
 
 namespace {
 thread_local std::string g_error;
+thread_local int g_cu_budget = 0;
 }
 
 void set_error(const std::string& msg) { g_error = msg; }
 const char* last_error() { return g_error.c_str(); }
+int& cu_budget() { return g_cu_budget; }
 
 namespace {
 
@@ -114,6 +116,12 @@ using namespace ggp;
 extern "C" {
 
 const char* ggp_last_error(void) { return ggp::last_error(); }
+
+int ggp_set_cu_budget(int cus) {
+  const int prev = ggp::cu_budget();
+  ggp::cu_budget() = cus > 0 ? cus : 0;
+  return prev;
+}
 int ggp_abi_version(void) { return 1; }
 
 size_t ggp_build_tree_workspace(int64_t n, int d, int dtype) { return build_tree_workspace(n, d, dtype); }

@@ -343,8 +343,8 @@ int launch_adjoint_config(AdjointArgs<T> a, hipStream_t s, size_t lds, int grid)
     GGP_HIP(hipFuncSetAttribute((const void*)k1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     GGP_HIP(hipFuncSetAttribute((const void*)k0, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   }
-  hipLaunchKernelGGL(k1, dim3(grid), dim3(256), lds, s, a);
-  hipLaunchKernelGGL(k0, dim3(grid), dim3(256), lds, s, a);
+  hipLaunchKernelGGL(k1, dim3(persistent_grid((const void*)k1, grid, lds)), dim3(256), lds, s, a);
+  hipLaunchKernelGGL(k0, dim3(persistent_grid((const void*)k0, grid, lds)), dim3(256), lds, s, a);
   GGP_LAUNCH_CHECK();
   return kOk;
 }

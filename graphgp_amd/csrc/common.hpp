@@ -152,4 +152,19 @@ int fill_u64(void* p, uint64_t v, int64_t n, hipStream_t s);
 
 inline int dtype_size(int dtype) { return dtype == 1 ? 8 : 4; }  // 0 = float32, 1 = float64
 
+// CU budget of the calling host thread for persistent (work-queue) launches; 0 = all CUs.
+int& cu_budget();
+
+// Grid of a persistent launch: at most `grid` workgroups and, under a CU budget, at most
+// budget x (workgroups of `kernel` resident per CU), so the launch leaves room elsewhere.
+inline int persistent_grid(const void* kernel, int grid, size_t lds_bytes) {
+  const int budget = cu_budget();
+  if (budget <= 0) return grid;
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, lds_bytes) != hipSuccess || per_cu < 1)
+    per_cu = 1;
+  const int cap = budget * per_cu;
+  return grid < cap ? grid : cap;
+}
+
 }  // namespace ggp

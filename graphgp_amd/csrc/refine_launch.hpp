@@ -16,6 +16,7 @@ int launch_variant(const RefineArgs<T>& a, hipStream_t s, size_t lds_bytes, int 
       return kUnsupported;
     }
   }
+  grid = persistent_grid((const void*)kern, grid, lds_bytes);
   hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds_bytes, s, a);
   GGP_LAUNCH_CHECK();
   return kOk;

@@ -184,6 +184,13 @@ int ggp_refine_counters(const void* workspace, uint32_t* host_counters, void* st
  * switch off. */
 int ggp_profile_events(void* start_event, void* stop_event);
 
+/* CU budget (no reference counterpart; for a serving loop that runs small and large
+ * requests concurrently): the persistent refinement and adjoint launches issued by the
+ * calling host thread use at most `cus` CUs' worth of resident workgroups (0 = all CUs),
+ * leaving room for other streams' kernels.  Results do not depend on it.  Returns the
+ * previous value. */
+int ggp_set_cu_budget(int cus);
+
 #ifdef __cplusplus
 }
 #endif
