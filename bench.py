@@ -63,9 +63,112 @@ def algorithmic_flops(k, d):
     return pairs * (3 * d) + pairs * 3 + k ** 3 / 3 + 2 * k * k + 4 * k
 
 
+def run_differentiation(args):
+    """The differentiation run: N=4,194,304 uniform in [0,1]^3, n0=500, k=20, RBF (variance 1,
+    scale 0.1) on r=0 plus 1000 log bins over [1e-5, 2], jitter 1e-5.  L = sum w_i values_i
+    with w ~ N(0,1).  Times generate and its vector-Jacobian product (adjoint kernel + FP64
+    seed-block adjoint) in FP64 and FP32, and checks the gradients: the dot-product identity
+    <w, J u> = <J^T w, u> (generate is linear in xi), central differences with h=1e-3 on 64
+    random xi coordinates and with h=1e-8 x |value| on 64 table entries with non-negligible
+    gradient (the loss is strongly curved in the table: at this length scale the local Schur
+    complements are tiny, so h=1e-6 x |value| already carries ~1% truncation error and
+    h=1e-4 x |value| makes some local systems indefinite), log-det, and the inverse round
+    trip."""
+    import torch
+
+    import graphgp_amd as gp
+
+    n, n0, k = args.n, 500, 20
+    rng = np.random.default_rng(3)
+    pts = torch.from_numpy(rng.uniform(0, 1, (n, 3)).astype(np.float32)).cuda()
+    cov = gp.extras.rbf_kernel(variance=1.0, scale=0.1, r_min=1e-5, r_max=2.0, n_bins=1000, jitter=1e-5)
+    bins, vals = cov
+    torch.cuda.synchronize()
+    t = time.time()
+    g = gp.build_graph(pts, n0=n0, k=k)
+    torch.cuda.synchronize()
+    build_ms = (time.time() - t) * 1e3
+    xi = torch.from_numpy(rng.standard_normal(n)).cuda()
+    w = torch.from_numpy(rng.standard_normal(n)).cuda()
+    n_ref = n - n0
+
+    def timed(fn, steps):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(steps):
+            out = fn()
+        b.record()
+        torch.cuda.synchronize()
+        return out, a.elapsed_time(b) / steps
+
+    res = {}
+    for prec in ("fp64", "fp32"):
+        x = xi if prec == "fp64" else xi.float()
+        ww = w if prec == "fp64" else w.float()
+        v, fwd_ms = timed(lambda: gp.generate(g, cov, x, precision=prec), args.steps)
+        (gx, gv), vjp_ms = timed(lambda: gp.generate_vjp(g, cov, x, v, ww, precision=prec), args.steps)
+        res[prec] = dict(v=v, gx=gx, gv=gv, fwd_ms=fwd_ms, vjp_ms=vjp_ms)
+        log(f"{prec}: generate {fwd_ms:.2f} ms, vjp {vjp_ms:.2f} ms")
+    gx64, gv64 = res["fp64"]["gx"], res["fp64"]["gv"]
+    u = torch.from_numpy(np.random.default_rng(4).standard_normal(n)).cuda()
+    lhs = float((w * gp.generate(g, cov, u, precision="fp64")).sum())
+    rhs = float((gx64 * u).sum())
+    loss = lambda x_, vv: float((w * gp.generate(g, (bins, vv), x_, precision="fp64")).sum())  # noqa: E731
+    fd_xi = 0.0
+    for j in np.random.default_rng(5).choice(n, 64, replace=False):
+        e = torch.zeros_like(xi)
+        e[j] = 1e-3
+        fd = (loss(xi + e, vals) - loss(xi - e, vals)) / 2e-3
+        fd_xi = max(fd_xi, abs(fd - float(gx64[j])) / max(abs(float(gx64[j])), 1e-3 * float(gx64.abs().max())))
+    nz = torch.nonzero(gv64.abs() > 1e-3 * gv64.abs().max()).flatten().cpu().numpy()
+    fd_tab = 0.0
+    for j in np.random.default_rng(6).choice(nz, min(64, len(nz)), replace=False):
+        h = 1e-8 * max(abs(float(vals[j])), 1e-6)
+        e = torch.zeros_like(vals)
+        e[j] = h
+        fd = (loss(xi, vals + e) - loss(xi, vals - e)) / (2 * h)
+        fd_tab = max(fd_tab, abs(fd - float(gv64[j])) / abs(float(gv64[j])))
+    ld64 = float(gp.generate_logdet(g, cov, precision="fp64"))
+    ld32 = float(gp.generate_logdet(g, cov, precision="fp32"))
+    xb = gp.generate_inv(g, cov, res["fp64"]["v"], precision="fp64")
+    rt = float((xb - xi).abs().max())
+    gx32, gv32 = res["fp32"]["gx"], res["fp32"]["gv"]
+    line = {
+        "metric": "refined points/s of the vector-Jacobian product of generate (adjoint kernel)",
+        "value": round(n_ref / (res["fp32"]["vjp_ms"] * 1e-3), 1),
+        "unit": "refined points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "higher_is_better": True,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "differentiation: N=4,194,304 uniform [0,1]^3, n0=500, k=20, RBF scale 0.1, "
+                               "1000 log bins over [1e-5, 2], jitter 1e-5, L = sum w*values",
+                   "n": n, "k": k, "n0": n0},
+        "build_graph_ms": round(build_ms, 2),
+        "fp64": {"generate_ms": round(res["fp64"]["fwd_ms"], 3), "vjp_ms": round(res["fp64"]["vjp_ms"], 3)},
+        "fp32": {"generate_ms": round(res["fp32"]["fwd_ms"], 3), "vjp_ms": round(res["fp32"]["vjp_ms"], 3)},
+        "checks": {
+            "dot_product_rel": abs(lhs - rhs) / abs(lhs),
+            "fd_xi_h1e-3_64coords_max_rel": fd_xi,
+            "fd_table_h1e-8rel_64entries_max_rel": fd_tab,
+            "fp32_vs_fp64_xi_bar_max_rel": float((gx32.double() - gx64).abs().max() / gx64.abs().max()),
+            "fp32_vs_fp64_table_bar_rel_l2": float((gv32 - gv64).norm() / gv64.norm()),
+            "logdet_fp64": ld64,
+            "logdet_fp32_rel": abs(ld32 - ld64) / abs(ld64),
+            "inverse_round_trip_max_abs_fp64": rt,
+        },
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", choices=["headline", "differentiation"], default="headline",
+                    help="headline: the generate benchmark (default); differentiation: the adjoint run")
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=16_777_216)
@@ -80,6 +183,10 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); every box in this setup has one GPU",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "differentiation":
+        if args.n == 16_777_216:
+            args.n = 4_194_304
+        return run_differentiation(args)
 
     import torch
 

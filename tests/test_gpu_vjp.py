@@ -101,14 +101,16 @@ def test_autograd_matches_vjp():
     loss = (out * torch.tensor(w, device="cuda")).sum()
     loss.backward()
     gx, gv = gp.generate_vjp(g, (bins, tv.detach()), xi, out.detach(), w, precision="fp64")
-    assert torch.allclose(x.grad, gx, rtol=1e-12, atol=0)
-    assert torch.allclose(tv.grad, gv, rtol=1e-12, atol=0)
+    # neighbour cotangents are summed with atomics, so two backward passes agree to the
+    # rounding of those sums: near-zero components get an absolute tolerance
+    assert torch.allclose(x.grad, gx, rtol=1e-10, atol=1e-12 * float(gx.abs().max()))
+    assert torch.allclose(tv.grad, gv, rtol=1e-10, atol=1e-12 * float(gv.abs().max()))
     # a second backward pass gives the same gradients (no state kept between calls); the
     # cotangents of shared neighbours are summed with atomics, so only the order of those
     # additions (last bits) may differ
     x2 = torch.tensor(xi, device="cuda", requires_grad=True)
     (gp.generate(g, (bins, tv.detach()), x2, precision="fp64") * torch.tensor(w, device="cuda")).sum().backward()
-    assert torch.allclose(x2.grad, x.grad, rtol=1e-12, atol=1e-14)
+    assert torch.allclose(x2.grad, x.grad, rtol=1e-10, atol=1e-12 * float(x.grad.abs().max()))
 
 
 def test_vjp_clustered_k30():
