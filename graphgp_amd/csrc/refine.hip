@@ -384,7 +384,28 @@ struct RefineWs {
   double* dvec = nullptr;  // n0
   double* dout = nullptr;  // n0 (+2 scalars)
   T* scratch = nullptr;
+  // two-phase generate (FP32 increment mode)
+  T* wts = nullptr;
+  T* sig = nullptr;
+  int32_t* fixslot = nullptr;
+  uint32_t* fix_list = nullptr;
+  uint32_t fix_cap = 0;
+  double* w64 = nullptr;
+  double* sig64 = nullptr;
 };
+
+inline bool generic_config(const RefineProblem& p) { return p.k + 1 > 40 || p.d > 8 || (p.d > 3 && p.k + 1 > 16); }
+
+// generate in the FP32 increment mode runs in three phases (weights, FP64 recomputation of
+// the flagged points, dataflow over the stored weights)
+template <typename T>
+bool two_phase(const RefineProblem& p, int mode) {
+  return sizeof(T) == 4 && p.increment && mode == kGenerate && !generic_config(p) && p.n_ref > 0;
+}
+
+// list slots for the FP64 recomputation: a quarter of the points (the workloads measured here
+// flag about 1%); beyond that the extra points keep their FP32 weights and error bit 2 is set
+inline uint32_t fix_capacity(int64_t n_ref) { return (uint32_t)std::max<int64_t>(4096, n_ref / 4); }
 
 template <typename T>
 bool carve(Workspace& w, const RefineProblem& p, int mode, RefineWs<T>& b) {
@@ -405,7 +426,16 @@ bool carve(Workspace& w, const RefineProblem& p, int mode, RefineWs<T>& b) {
     b.dvec = w.take<double>(n0 + 1);
     b.dout = w.take<double>(n0 + 4);
   }
-  if (p.k + 1 > 40 || p.d > 8 || (p.d > 3 && p.k + 1 > 16)) b.scratch = w.take<T>(generic_scratch_bytes<T>(p.k) / sizeof(T));
+  if (generic_config(p)) b.scratch = w.take<T>(generic_scratch_bytes<T>(p.k) / sizeof(T));
+  if (two_phase<T>(p, mode)) {
+    b.fix_cap = (uint32_t)std::min<int64_t>(fix_capacity(p.n_ref), p.n_ref);
+    b.wts = w.take<T>((size_t)p.n_ref * p.k);
+    b.sig = w.take<T>(p.n_ref);
+    b.fixslot = w.take<int32_t>(p.n_ref);
+    b.fix_list = w.take<uint32_t>(b.fix_cap);
+    b.w64 = w.take<double>((size_t)b.fix_cap * p.k);
+    b.sig64 = w.take<double>(b.fix_cap);
+  }
   return w.ok();
 }
 
@@ -428,8 +458,9 @@ int run_kernel(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hipStrea
   return rc;
 }
 
+// Problem and workspace fields of the kernel arguments (the caller sets mode, values, I/O).
 template <typename T>
-int run_kernel_inner(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hipStream_t s) {
+void setup_args(RefineArgs<T>& a, const RefineProblem& p, RefineWs<T>& b) {
   a.pts = p.pts;
   a.pts_f64 = p.pts_f64;
   a.d = p.d;
@@ -447,6 +478,18 @@ int run_kernel_inner(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hi
   a.meta64 = b.meta64;
   a.fix_count = b.ctr + 3;
   a.fix_tau = fix_tau();
+  a.wts = b.wts;
+  a.sig = b.sig;
+  a.fixslot = b.fixslot;
+  a.fix_list = b.fix_list;
+  a.fix_cap = b.fix_cap;
+  a.w64 = b.w64;
+  a.sig64 = b.sig64;
+}
+
+template <typename T>
+int run_kernel_inner(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hipStream_t s) {
+  setup_args<T>(a, p, b);
   if (p.n_ref == 0) return kOk;
   const int kk = p.k + 1;
   const bool generic = kk > 40 || p.d > 8 || (p.d > 3 && kk > 16);
@@ -463,6 +506,40 @@ int run_kernel_inner(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hi
   int rc = launch_refine<T>(a, kk, s, lds, grid);
   if (rc == kUnsupported && ggp::last_error()[0] == 0) set_error("refine: unsupported configuration");
   return rc;
+}
+
+// The three phases of generate in the FP32 increment mode (a.vals already holds the seeds
+// and sentinels): the weights kernel, the FP64 recomputation of the listed points, and the
+// dataflow over the stored weights.
+template <typename T>
+int run_two_phase(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hipStream_t s) {
+  if (g_prof_start) GGP_HIP(hipEventRecord(g_prof_start, s));
+  setup_args<T>(a, p, b);  // the later phases launch with this copy of the arguments
+  a.mode = kWeights;
+  GGP_TRY(run_kernel_inner<T>(a, p, b, s));
+  const int cus = device_cus();
+  const size_t lds = (size_t)4 * (p.k + 1) * (p.k + 2) * sizeof(double);
+  auto launch_fix = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(persistent_grid((const void*)kern, cus * 4, lds)), dim3(256), lds, s, a);
+  };
+  if (p.d <= 3) {
+    launch_fix(fix_list_kernel<T, 3, true>);
+    launch_fix(fix_list_kernel<T, 3, false>);
+  } else {
+    launch_fix(fix_list_kernel<T, 8, true>);
+    launch_fix(fix_list_kernel<T, 8, false>);
+  }
+  GGP_LAUNCH_CHECK();
+  GGP_TRY(fill_u32(b.ctr, 0u, 1, s));  // fresh ticket for the dataflow
+  a.mode = kGenerate;
+  constexpr int P = 4, RM = 10;  // k <= 40 for the register configurations
+  a.ntiles = (p.n_ref + 256 / P - 1) / (256 / P);
+  auto kern = apply_kernel<T, P, RM>;
+  const int grid = (int)std::min<int64_t>(a.ntiles, persistent_grid((const void*)kern, cus * 8, 0));
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, s, a);
+  GGP_LAUNCH_CHECK();
+  if (g_prof_stop) GGP_HIP(hipEventRecord(g_prof_stop, s));
+  return kOk;
 }
 
 template <typename T>
@@ -513,7 +590,8 @@ int refine_flow(const RefineProblem& p, int mode, const RefineIO& io, void* ws, 
     a.xi = io.xi;
     a.xi_f64 = io.xi_f64;
     a.out_f64 = io.out_f64;
-    GGP_TRY(run_kernel<T>(a, p, b, s));
+    if (two_phase<T>(p, mode)) GGP_TRY(run_two_phase<T>(a, p, b, s));
+    else GGP_TRY(run_kernel<T>(a, p, b, s));
     if (p.with_seed) {
       hipLaunchKernelGGL(nan_fill_kernel, dim3(grid_for(std::min<int64_t>(p.n, 1 << 20), 256)), dim3(256), 0, s,
                          io.out, io.out_f64, p.n, b.ctr + 1);

@@ -35,7 +35,8 @@ __device__ __forceinline__ int log_guess(float r2, float lb1, float inv_step) {
   return 1 + (int)((lr - lb1) * inv_step);
 }
 
-enum RefineMode : int { kGenerate = 0, kInverse = 1, kLogdet = 2 };
+// kWeights: first phase of the two-phase generate (per-point weights and sigma, no waiting)
+enum RefineMode : int { kGenerate = 0, kInverse = 1, kLogdet = 2, kWeights = 3 };
 
 template <typename T>
 struct RefineArgs {
@@ -68,6 +69,17 @@ struct RefineArgs {
   double fix_tau;
   uint32_t* fix_count;
   int fix_lds_offset;
+  // two-phase generate: weights w (n_ref x k) and sigma of every point (mean = w . u, with
+  // u the neighbour values, in increments of neighbour 0 in the increment formulation);
+  // the points needing the FP64 recomputation, their list slot (-1: none), FP64 weights
+  // and sigma
+  T* wts;
+  T* sig;
+  int32_t* fixslot;
+  uint32_t* fix_list;
+  uint32_t fix_cap;
+  double* w64;
+  double* sig64;
 };
 
 // ------------------------------------------------------------------ helpers
@@ -453,7 +465,7 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
     // FP64 recomputation of the tile's flagged points before any waiting: the flagged
     // groups are listed in LDS and the workgroup's four wavefronts take them in turn
     double sig64 = 1.0;
-    if (fixing) {
+    if (fixing && a.mode != kWeights) {
       if (threadIdx.x == 0) s_nfix = 0;
       __syncthreads();
       if (flagged && lane == 0) s_fixlist[atomicAdd(&s_nfix, 1)] = grp;
@@ -471,6 +483,53 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
       __syncthreads();
       if (flagged) sig64 = s_fixsig[grp];
     }
+    if (a.mode == kWeights) {
+      // w = L_A^{-T} z with z = L[k, :k] (so that mean = z . L_A^{-1} u = w . u): with
+      // w_k = -1, w_i = -(sum_{i<m<=k} L[m][i] w_m) / L[i][i], rows m spread over the
+      // group's lanes, one group sum per column
+      T wv[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) wv[r] = (P * r + lane == k) ? T(-1) : T(0);
+      static_for<0, KK - 1>([&](auto ic) {
+        constexpr int i = KK - 2 - decltype(ic)::value;
+        constexpr int is = i / P, il = i % P;
+        if (i < k) {
+          T part = T(0);
+#pragma unroll
+          for (int r = is; r < R; ++r) {
+            const int row = P * r + lane;
+            if (row > i && row <= k) part += m[P * r * (r + 1) / 2 + i] * wv[r];
+          }
+#pragma unroll
+          for (int o = 1; o < P; o <<= 1) part += __shfl_xor(part, o, 64);
+          const T wi = -part * bcast<P, il>(rdl[is]);
+          if (lane == il) wv[is] = wi;
+        }
+      });
+      if (valid) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          if (P * r + lane < k) a.wts[pt * k + P * r + lane] = wv[r];
+        if (owner) {
+          // a NaN here is not yet an error: a flagged point's FP32 sigma is replaced by its
+          // FP64 recomputation, and the published values are checked in the last phase
+          a.sig[pt] = sqrt(skk);
+          int32_t slot = -1;
+          if (flagged) {
+            const uint32_t q = atomicAdd(a.fix_count, 1u);
+            if (q < a.fix_cap) {
+              a.fix_list[q] = (uint32_t)pt;
+              slot = (int32_t)q;
+            } else {
+              atomicOr(&a.flags[1], 2u);  // more points to recompute than list slots
+            }
+          }
+          a.fixslot[pt] = slot;
+        }
+      }
+      continue;
+    }
+
     // FP64 mean of a recomputed point from this lane's rows' raw neighbour values
     auto fixed_mean = [&](const T (&uv)[R]) -> double {
       double part = 0.0;
@@ -615,6 +674,140 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
         if (!done) {
           atomicOr(&a.flags[1], 1u);
           atomicOr(&a.flags[0], 1u);  // the whole output becomes NaN
+          if (owner) __hip_atomic_store(&a.vals[fine], (T)__builtin_nan(""), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        break;
+      }
+      __builtin_amdgcn_s_sleep(8);
+    }
+  }
+}
+
+// Second phase of the two-phase generate: the FP64 recomputation of the listed points, one
+// wavefront per point, fully parallel (the weights do not depend on any value).
+template <typename T, int D, bool FAST>
+__global__ __launch_bounds__(256) void fix_list_kernel(RefineArgs<T> a) {
+  extern __shared__ __attribute__((aligned(16))) char smem_raw[];
+  if ((a.meta64->log_exact != 0) != FAST) return;
+  const int ld = a.k + 2;
+  double* slab = reinterpret_cast<double*>(smem_raw) + (threadIdx.x >> 6) * (a.k + 1) * ld;
+  uint32_t count = __hip_atomic_load(a.fix_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (count > a.fix_cap) count = a.fix_cap;
+  for (uint32_t e = blockIdx.x * 4 + (threadIdx.x >> 6); e < count; e += gridDim.x * 4) {
+    const int64_t pt = a.fix_list[e];
+    const double sg = fix64<T, D, FAST>(a, pt, true, slab, ld, a.w64 + (int64_t)e * a.k);
+    if ((threadIdx.x & 63) == 0) a.sig64[e] = sg;
+  }
+}
+
+// Third phase: the dataflow over the stored weights, P lanes per point, lane j % P gathers
+// neighbour j.  A point waits until all its neighbours' values are published (sentinel =
+// unpublished; a cached read first, the coherent re-read only while it is still the
+// sentinel), then publishes v = [v0] + w . u + sigma xi (FP64 weights and sum for the
+// recomputed points).  Persistent workgroups take tiles from an atomic ticket in index order,
+// so every dependency is on a lower ticket held by a running workgroup.
+template <typename T, int P, int RM>
+__global__ __launch_bounds__(256) void apply_kernel(RefineArgs<T> a) {
+  constexpr int GPT = 256 / P;
+  __shared__ uint32_t s_tile;
+  const int lane = threadIdx.x & (P - 1);
+  const int grp = threadIdx.x / P;
+  const int k = a.k;
+  const bool incr = a.increment != 0;
+  const int gbase = (threadIdx.x & 63) & ~(P - 1);
+  const uint64_t gmask = (P == 64) ? ~0ull : ((1ull << P) - 1ull);
+  for (;;) {
+    if (threadIdx.x == 0) s_tile = atomicAdd(a.ticket, 1u);
+    __syncthreads();
+    const int64_t tile = s_tile;
+    __syncthreads();
+    if (tile >= a.ntiles) break;
+    const int64_t pt = tile * GPT + grp;
+    const bool valid = pt < a.n_ref;
+    const int64_t fine = a.n0 + pt;
+    const int32_t slot = valid ? a.fixslot[pt] : -1;
+    int32_t nb[RM];
+    double wd[RM];
+    uint32_t pend = 0;
+#pragma unroll
+    for (int r = 0; r < RM; ++r) {
+      const int j = P * r + lane;
+      nb[r] = 0;
+      wd[r] = 0.0;
+      if (valid && j < k) {
+        nb[r] = a.nbrs[pt * k + j];
+        wd[r] = slot >= 0 ? a.w64[(int64_t)slot * k + j] : (double)a.wts[pt * k + j];
+        pend |= 1u << r;
+      }
+    }
+    T xi = T(0), sg = T(0);
+    int64_t dst = fine;
+    const bool owner = lane == 0;
+    if (valid && owner) {
+      dst = a.perm ? (int64_t)a.perm[fine] : fine;
+      xi = load_as<T>(a.xi, a.xi_f64, dst - a.io_base);
+      sg = slot >= 0 ? (T)0 : a.sig[pt];
+    }
+    T u[RM];
+#pragma unroll
+    for (int r = 0; r < RM; ++r) u[r] = T(0);
+    bool done = !valid;
+    for (uint32_t it = 0;; ++it) {
+      if (!done) {
+#pragma unroll
+        for (int r = 0; r < RM; ++r) {
+          if (pend & (1u << r)) {
+            T v = a.vals[nb[r]];
+            if (Sentinel<T>::is(v)) v = __hip_atomic_load(&a.vals[nb[r]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (!Sentinel<T>::is(v)) {
+              u[r] = v;
+              pend &= ~(1u << r);
+            }
+          }
+        }
+        const uint64_t bal = __ballot(pend == 0);
+        if (((bal >> gbase) & gmask) == gmask) {
+          T v;
+          if (slot >= 0) {
+            double part = 0.0;
+#pragma unroll
+            for (int r = 0; r < RM; ++r) part += wd[r] * (double)u[r];
+#pragma unroll
+            for (int o = 1; o < P; o <<= 1) part += __shfl_xor(part, o, 64);
+            v = (T)(part + a.sig64[slot] * (double)xi);
+          } else {
+            // neighbour 0 is lane 0's first slot
+            const T v0 = incr ? __shfl(u[0], gbase, 64) : T(0);
+            T part = T(0);
+#pragma unroll
+            for (int r = 0; r < RM; ++r) {
+              const int j = P * r + lane;
+              const T uj = (incr && j != 0) ? u[r] - v0 : u[r];
+              part += (T)wd[r] * uj;
+            }
+#pragma unroll
+            for (int o = 1; o < P; o <<= 1) part += __shfl_xor(part, o, 64);
+            v = v0 + part + sg * xi;
+          }
+          if (owner) {
+            if (v != v) {
+              atomicOr(&a.flags[0], 1u);
+              v = __builtin_nan("");
+            }
+            __hip_atomic_store(&a.vals[fine], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (a.out) {
+              if (a.out_f64) ((double*)a.out)[dst] = (double)v;
+              else ((float*)a.out)[dst] = (float)v;
+            }
+          }
+          done = true;
+        }
+      }
+      if (__all(done)) break;
+      if (it > (1u << 22)) {
+        if (!done) {
+          atomicOr(&a.flags[1], 1u);
+          atomicOr(&a.flags[0], 1u);
           if (owner) __hip_atomic_store(&a.vals[fine], (T)__builtin_nan(""), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         break;
