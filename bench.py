@@ -164,11 +164,98 @@ def run_differentiation(args):
     print(json.dumps(line), flush=True)
 
 
+def run_billion(args):
+    """The billion-point run: N uniform in [0,1]^3 (default 1e9), n0=1000, k=16, RBF (variance
+    1, scale 0.01) on r=0 plus 4000 log bins over [1e-6, 2], jitter 1e-5, xi ~ N(0,1) float32,
+    int32 neighbour indices.  build_graph runs stage by stage (tree, kNN, depths, order,
+    offsets) so that each stage's scratch is freed before the next (the graph alone is ~85 GB
+    at 1e9), then one FP32 generate and one FP64 log-det.  Per stage: time, the stage's
+    algorithmic bytes / time, and for the refinement its flops / time against the VALU peak."""
+    import torch
+
+    import graphgp_amd as gp
+    from graphgp_amd import _lib
+    from graphgp_amd.graph import _offsets
+
+    n, n0, k, d = args.n, 1000, 16, 3
+    log(f"billion workload N={n} k={k}")
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(2025)
+    pts = torch.rand((n, d), generator=gen, device="cuda", dtype=torch.float32)
+    cov = gp.extras.rbf_kernel(variance=1.0, scale=0.01, r_min=1e-6, r_max=2.0, n_bins=4001, jitter=1e-5)
+    n_ref = n - n0
+    stages = {}
+
+    def stage(name, fn, nbytes):
+        torch.cuda.synchronize()
+        t = time.time()
+        out = fn()
+        torch.cuda.synchronize()
+        dt = time.time() - t
+        stages[name] = {"s": round(dt, 4), "algorithmic_GBs": round(nbytes / dt / 1e9, 1)}
+        log(f"{name}: {dt:.3f} s, {torch.cuda.max_memory_allocated() / 2**30:.1f} GiB peak so far")
+        return out
+
+    levels = int(math.floor(math.log2(n))) + 1
+    tp, sd, idx = stage("tree", lambda: gp.build_tree(pts), 2 * n * (d * 4 + 4) * levels)
+    del pts
+    torch.cuda.empty_cache()
+    nb = stage("knn", lambda: gp.query_preceding_neighbors(tp, sd, n0=n0, k=k), n * d * 4 + n_ref * k * 4)
+    del sd
+    torch.cuda.empty_cache()
+    dep = stage("depths", lambda: gp.compute_depths(nb, n0=n0), n_ref * k * 20)
+    torch.cuda.empty_cache()
+    po, io, no, do = stage("order", lambda: gp.order_by_depth(tp, idx, nb, dep),
+                           2 * (n * (d * 4 + 8) + n_ref * k * 4) + n_ref * k * 4)
+    del tp, idx, nb, dep
+    torch.cuda.empty_cache()
+    offsets = stage("offsets", lambda: _offsets(do, int(do[-1].item())), n * 4)
+    g = gp.Graph(po, no, offsets, io)
+    del do
+    torch.cuda.empty_cache()
+    xi = torch.randn(n, generator=gen, device="cuda", dtype=torch.float32)
+    L = _lib.lib()
+    ks, ke = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ks.record()
+    ke.record()
+    L.ggp_profile_events(ks.cuda_event, ke.cuda_event)
+    v = stage("generate_fp32", lambda: gp.generate(g, cov, xi, precision="fp32"), algorithmic_bytes(k, d, 4, 4) * n_ref)
+    L.ggp_profile_events(None, None)
+    kernel_s = ks.elapsed_time(ke) * 1e-3
+    nan = bool(torch.isnan(v).any())
+    del v
+    torch.cuda.empty_cache()
+    ld = stage("logdet_fp64", lambda: float(gp.generate_logdet(g, cov, precision="fp64")), n_ref * (4 * k + 12 * k))
+    tflops = algorithmic_flops(k, d) * n_ref / kernel_s / 1e12
+    build_s = sum(stages[s]["s"] for s in ("tree", "knn", "depths", "order", "offsets"))
+    line = {
+        "metric": "billion-point field: build_graph + FP32 generate + FP64 log-det",
+        "value": round(n_ref / stages["generate_fp32"]["s"], 1),
+        "unit": "refined points/s (generate)",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "N uniform [0,1]^3, n0=1000, k=16, RBF scale 0.01, 4000 log bins over [1e-6, 2], "
+                               "jitter 1e-5", "n": n, "k": k, "n0": n0, "n_batches": len(offsets) - 1},
+        "stages": stages,
+        "build_graph_s": round(build_s, 3),
+        "generate_refine_kernel_s": round(kernel_s, 4),
+        "generate_valu_TFLOPs": round(tflops, 2),
+        "generate_valu_frac": round(tflops / PEAK_FP32_TFLOPS, 4),
+        "logdet_fp64": ld,
+        "generate_has_nan": nan,
+        "peak_device_GiB": round(torch.cuda.max_memory_allocated() / 2**30, 1),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["headline", "differentiation"], default="headline",
-                    help="headline: the generate benchmark (default); differentiation: the adjoint run")
+    ap.add_argument("--workload", choices=["headline", "differentiation", "billion"], default="headline",
+                    help="headline: the generate benchmark (default); differentiation: the adjoint run; "
+                         "billion: the billion-point field (use --n to scale it)")
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=16_777_216)
@@ -187,6 +274,10 @@ def main():
         if args.n == 16_777_216:
             args.n = 4_194_304
         return run_differentiation(args)
+    if args.workload == "billion":
+        if args.n == 16_777_216:
+            args.n = 1_000_000_000
+        return run_billion(args)
 
     import torch
 

@@ -39,8 +39,8 @@ __global__ void gather_kernel(const void* __restrict__ src, int src_f64, const i
 }
 
 __global__ void count_children_kernel(const int32_t* __restrict__ nbrs, int64_t m, uint32_t* __restrict__ cnt) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < m) atomicAdd(&cnt[nbrs[e]], 1u);
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += (int64_t)gridDim.x * blockDim.x)
+    atomicAdd(&cnt[nbrs[e]], 1u);
 }
 
 template <typename T>
@@ -243,7 +243,7 @@ int vjp_flow(const RefineProblem& p, const VjpIO& io, void* ws, size_t ws_bytes,
   GGP_TRY(fill_u32(b.children, 0u, p.n, s));
   GGP_TRY(fill_u64(io.gtab, 0ull, p.nb, s));
   if (p.n_ref > 0) {
-    hipLaunchKernelGGL(count_children_kernel, dim3(grid_for(p.n_ref * p.k, 256)), dim3(256), 0, s, p.nbrs,
+    hipLaunchKernelGGL(count_children_kernel, dim3(grid_capped(p.n_ref * p.k, 256)), dim3(256), 0, s, p.nbrs,
                        p.n_ref * (int64_t)p.k, b.children);
     GGP_LAUNCH_CHECK();
     AdjointArgs<T> a{};

@@ -143,6 +143,13 @@ struct KeyOf<double> {
 
 // ------------------------------------------------------------------ misc
 inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+// Grid of a grid-stride launch over n items: a dispatch's size in work-items is a 32-bit
+// field, so one-thread-per-item launches must stay below 2^32 threads; 2^20 workgroups is
+// plenty to fill the device.
+inline unsigned grid_capped(int64_t n, int block) {
+  const int64_t g = (n + block - 1) / block;
+  return (unsigned)(g < (int64_t(1) << 20) ? g : (int64_t(1) << 20));
+}
 
 // Fill a buffer with a 32-bit pattern (used instead of hipMemsetAsync so that a
 // captured graph never holds memset nodes).

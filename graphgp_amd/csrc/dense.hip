@@ -38,33 +38,34 @@ __device__ inline double interp_dev(double x, const double* __restrict__ xp, con
 __global__ void dense_cov_kernel(const void* __restrict__ pts, int pts_f64, int d, int64_t n,
                                  const double* __restrict__ bins, const double* __restrict__ vals, int nb,
                                  double* __restrict__ A) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n * n) return;
-  int64_t i = e / n, j = e - i * n;
-  double s = 0.0;
-  for (int t = 0; t < d; ++t) {
-    double a = pts_f64 ? ((const double*)pts)[i * d + t] : (double)((const float*)pts)[i * d + t];
-    double b = pts_f64 ? ((const double*)pts)[j * d + t] : (double)((const float*)pts)[j * d + t];
-    double u = a - b;
-    s += u * u;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e / n, j = e - i * n;
+    double s = 0.0;
+    for (int t = 0; t < d; ++t) {
+      double a = pts_f64 ? ((const double*)pts)[i * d + t] : (double)((const float*)pts)[i * d + t];
+      double b = pts_f64 ? ((const double*)pts)[j * d + t] : (double)((const float*)pts)[j * d + t];
+      double u = a - b;
+      s += u * u;
+    }
+    A[e] = interp_dev(sqrt(s), bins, vals, nb);
   }
-  A[e] = interp_dev(sqrt(s), bins, vals, nb);
 }
 
 __global__ void cross_cov_kernel(const void* __restrict__ pa, int64_t na, const void* __restrict__ pb, int64_t nb_,
                                  int d, int f64, const double* __restrict__ bins, const double* __restrict__ vals,
                                  int nbins, double* __restrict__ out) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= na * nb_) return;
-  int64_t i = e / nb_, j = e - i * nb_;
-  double s = 0.0;
-  for (int t = 0; t < d; ++t) {
-    double a = f64 ? ((const double*)pa)[i * d + t] : (double)((const float*)pa)[i * d + t];
-    double b = f64 ? ((const double*)pb)[j * d + t] : (double)((const float*)pb)[j * d + t];
-    double u = a - b;
-    s += u * u;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < na * nb_;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e / nb_, j = e - i * nb_;
+    double s = 0.0;
+    for (int t = 0; t < d; ++t) {
+      double a = f64 ? ((const double*)pa)[i * d + t] : (double)((const float*)pa)[i * d + t];
+      double b = f64 ? ((const double*)pb)[j * d + t] : (double)((const float*)pb)[j * d + t];
+      double u = a - b;
+      s += u * u;
+    }
+    out[e] = interp_dev(sqrt(s), bins, vals, nbins);
   }
-  out[e] = interp_dev(sqrt(s), bins, vals, nbins);
 }
 
 // Factorise the diagonal block D (nbj x nbj, lower, in LDS) with 256 threads and write
@@ -279,7 +280,7 @@ size_t dense_workspace(int64_t n0) { return (size_t)(n0 * n0 + NB * NB) * sizeof
 int dense_factor(const void* pts, int pts_f64, int d, int64_t n0, const double* bins, const double* vals, int nb,
                  double* A, hipStream_t s) {
   if (n0 <= 0) return kOk;
-  hipLaunchKernelGGL(dense_cov_kernel, dim3(grid_for(n0 * n0, 256)), dim3(256), 0, s, pts, pts_f64, d, n0, bins,
+  hipLaunchKernelGGL(dense_cov_kernel, dim3(grid_capped(n0 * n0, 256)), dim3(256), 0, s, pts, pts_f64, d, n0, bins,
                      vals, nb, A);
   GGP_LAUNCH_CHECK();
   const int nblk = (int)((n0 + NB - 1) / NB);
@@ -312,7 +313,7 @@ int dense_solve(const double* L, int64_t n0, double* v, hipStream_t s) {
 int cov_matrix(const void* a, int64_t na, const void* b, int64_t nb_, int d, int f64, const double* bins,
                const double* vals, int nbins, double* out, hipStream_t s) {
   if (na * nb_ == 0) return kOk;
-  hipLaunchKernelGGL(cross_cov_kernel, dim3(grid_for(na * nb_, 256)), dim3(256), 0, s, a, na, b, nb_, d, f64, bins,
+  hipLaunchKernelGGL(cross_cov_kernel, dim3(grid_capped(na * nb_, 256)), dim3(256), 0, s, a, na, b, nb_, d, f64, bins,
                      vals, nbins, out);
   GGP_LAUNCH_CHECK();
   return kOk;

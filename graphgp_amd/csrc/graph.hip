@@ -98,12 +98,13 @@ __global__ void kahn_init_kernel(const int32_t* __restrict__ nbrs, int64_t n_ref
 __global__ void kahn_fill_kernel(const int32_t* __restrict__ nbrs, int64_t n_ref, int k, int64_t n0,
                                  const int64_t* __restrict__ rowptr, uint32_t* __restrict__ cur,
                                  uint32_t* __restrict__ child) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n_ref * k) return;
-  const int32_t nb = nbrs[e];
-  if (nb < n0) return;
-  const int64_t j = nb - n0;
-  child[rowptr[j] + atomicAdd(&cur[j], 1u)] = (uint32_t)(e / k);
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n_ref * k;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t nb = nbrs[e];
+    if (nb < n0) continue;
+    const int64_t j = nb - n0;
+    child[rowptr[j] + atomicAdd(&cur[j], 1u)] = (uint32_t)(e / k);
+  }
 }
 
 __global__ void kahn_start_kernel(uint32_t* ctr) {
@@ -245,12 +246,13 @@ __global__ void order_gather_kernel(const T* __restrict__ pts, int d, const int3
 __global__ void order_neighbors_kernel(const int32_t* __restrict__ nbrs, int k, const uint32_t* __restrict__ order,
                                        const uint32_t* __restrict__ inv, int64_t n0, int64_t n_ref,
                                        int32_t* __restrict__ nbrs_o) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n_ref * k) return;
-  const int64_t r = e / k;
-  const int j = (int)(e - r * k);
-  const int64_t src = (int64_t)order[n0 + r] - n0;
-  nbrs_o[e] = (int32_t)inv[nbrs[src * k + j]];
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n_ref * k;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = e / k;
+    const int j = (int)(e - r * k);
+    const int64_t src = (int64_t)order[n0 + r] - n0;
+    nbrs_o[e] = (int32_t)inv[nbrs[src * k + j]];
+  }
 }
 
 __global__ void offsets_kernel(const int32_t* __restrict__ depths_sorted, int64_t n, int64_t n_off,
@@ -385,7 +387,7 @@ int compute_depths(const int32_t* nbrs, int64_t n_ref, int k, int64_t n0, int32_
   hipLaunchKernelGGL(scan_tile_kernel, dim3((unsigned)tiles), dim3(256), 0, s, b.cnt, n_ref, b.sums, b.rowptr);
   GGP_LAUNCH_CHECK();
   GGP_TRY(fill_u32(b.cnt, 0u, n_ref, s));
-  hipLaunchKernelGGL(kahn_fill_kernel, dim3(grid_for(n_ref * k, 256)), dim3(256), 0, s, nbrs, n_ref, k, n0,
+  hipLaunchKernelGGL(kahn_fill_kernel, dim3(grid_capped(n_ref * k, 256)), dim3(256), 0, s, nbrs, n_ref, k, n0,
                      b.rowptr, b.cnt, b.child);
   hipLaunchKernelGGL(kahn_start_kernel, dim3(1), dim3(1), 0, s, b.ctr);
   GGP_LAUNCH_CHECK();
@@ -458,7 +460,7 @@ int order_by_depth(const void* pts, int64_t n, int d, int dtype, const int32_t* 
                        idx, depths, pay, n, (double*)pts_o, idx_o, dep_o, inv);
   GGP_LAUNCH_CHECK();
   if (n_ref > 0) {
-    hipLaunchKernelGGL(order_neighbors_kernel, dim3(grid_for(n_ref * k, 256)), dim3(256), 0, s, nbrs, k, pay, inv,
+    hipLaunchKernelGGL(order_neighbors_kernel, dim3(grid_capped(n_ref * k, 256)), dim3(256), 0, s, nbrs, k, pay, inv,
                        n0, n_ref, nbrs_o);
     GGP_LAUNCH_CHECK();
   }

@@ -518,7 +518,7 @@ int run_two_phase(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hipSt
   a.mode = kWeights;
   GGP_TRY(run_kernel_inner<T>(a, p, b, s));
   const int cus = device_cus();
-  const size_t lds = (size_t)4 * (p.k + 1) * (p.k + 2) * sizeof(double);
+  const size_t lds = (size_t)4 * (p.k + 1) * ((p.k + 2) | 1) * sizeof(double);  // odd row stride, as in the kernel
   auto launch_fix = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3(persistent_grid((const void*)kern, cus * 4, lds)), dim3(256), lds, s, a);
   };

@@ -689,7 +689,8 @@ template <typename T, int D, bool FAST>
 __global__ __launch_bounds__(256) void fix_list_kernel(RefineArgs<T> a) {
   extern __shared__ __attribute__((aligned(16))) char smem_raw[];
   if ((a.meta64->log_exact != 0) != FAST) return;
-  const int ld = a.k + 2;
+  // odd row stride (in doubles): the lanes' 64-bit row accesses then fall on distinct banks
+  const int ld = (a.k + 2) | 1;
   double* slab = reinterpret_cast<double*>(smem_raw) + (threadIdx.x >> 6) * (a.k + 1) * ld;
   uint32_t count = __hip_atomic_load(a.fix_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (count > a.fix_cap) count = a.fix_cap;

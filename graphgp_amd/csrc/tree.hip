@@ -209,11 +209,11 @@ __global__ void iota_kernel(uint32_t* p, uint32_t n) {
 template <typename T>
 __global__ void gather_points_kernel(const T* __restrict__ pts, int d, const int32_t* __restrict__ idx,
                                      T* __restrict__ out, int64_t n) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n * d) return;
-  int64_t i = e / d;
-  int j = (int)(e - i * d);
-  out[e] = pts[(int64_t)idx[i] * d + j];
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * d; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e / d;
+    const int j = (int)(e - i * d);
+    out[e] = pts[(int64_t)idx[i] * d + j];
+  }
 }
 
 struct TreeBufs {
@@ -326,7 +326,7 @@ static int build_tree_impl(const T* pts, int64_t n64, int d, T* pts_out, int32_t
     act = nxt;
     nxt = t;
   }
-  hipLaunchKernelGGL(gather_points_kernel<T>, dim3(grid_for(n64 * d, 256)), dim3(256), 0, s, pts, d, indices,
+  hipLaunchKernelGGL(gather_points_kernel<T>, dim3(grid_capped(n64 * d, 256)), dim3(256), 0, s, pts, d, indices,
                      pts_out, n64);
   GGP_LAUNCH_CHECK();
   return kOk;
