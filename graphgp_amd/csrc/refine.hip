@@ -517,19 +517,34 @@ int run_two_phase(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hipSt
   setup_args<T>(a, p, b);  // the later phases launch with this copy of the arguments
   a.mode = kWeights;
   GGP_TRY(run_kernel_inner<T>(a, p, b, s));
-  const int cus = device_cus();
-  const size_t lds = (size_t)4 * (p.k + 1) * ((p.k + 2) | 1) * sizeof(double);  // odd row stride, as in the kernel
-  auto launch_fix = [&](auto kern) {
-    hipLaunchKernelGGL(kern, dim3(persistent_grid((const void*)kern, cus * 4, lds)), dim3(256), lds, s, a);
-  };
-  if (p.d <= 3) {
-    launch_fix(fix_list_kernel<T, 3, true>);
-    launch_fix(fix_list_kernel<T, 3, false>);
-  } else {
-    launch_fix(fix_list_kernel<T, 8, true>);
-    launch_fix(fix_list_kernel<T, 8, false>);
+  // FP64 recomputation of the listed points with the FP64 register kernel (weights mode over
+  // the list, FP64 direct table)
+  {
+    GGP_TRY(fill_u32(b.ctr, 0u, 1, s));  // fresh ticket
+    RefineArgs<double> d{};
+    d.pts = p.pts;
+    d.pts_f64 = p.pts_f64;
+    d.d = p.d;
+    d.nbrs = p.nbrs;
+    d.k = p.k;
+    d.n0 = p.n - p.n_ref;
+    d.n_ref = p.n_ref;
+    d.tab = b.tab64;
+    d.meta = b.meta64;
+    d.ticket = b.ctr;
+    d.flags = b.ctr + 1;
+    d.mode = kWeights;
+    d.increment = 0;
+    d.fix_count = b.ctr + 3;
+    d.fix_cap = b.fix_cap;
+    d.w64 = b.w64;
+    d.sig64 = b.sig64;
+    d.list_in = b.fix_list;
+    const size_t lds64 = (size_t)(p.nb - 1) * sizeof(TabE<double>) + (size_t)p.nb * sizeof(double);
+    d.fix_lds_offset = (int)((lds64 + 15) & ~size_t(15));
+    GGP_TRY(launch_refine<double>(d, p.k + 1, s, lds64, device_cus() * 4));
   }
-  GGP_LAUNCH_CHECK();
+  const int cus = device_cus();
   GGP_TRY(fill_u32(b.ctr, 0u, 1, s));  // fresh ticket for the dataflow
   a.mode = kGenerate;
   constexpr int P = 4, RM = 10;  // k <= 40 for the register configurations

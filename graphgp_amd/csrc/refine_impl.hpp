@@ -80,6 +80,9 @@ struct RefineArgs {
   uint32_t fix_cap;
   double* w64;
   double* sig64;
+  // weights mode over a list: tile slot i is point list_in[i] for i < min(*fix_count,
+  // fix_cap); outputs go to w64 / sig64 at slot i
+  const uint32_t* list_in;
 };
 
 // ------------------------------------------------------------------ helpers
@@ -367,8 +370,16 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
     const int64_t tile = s_tile;
     __syncthreads();
     if (tile >= a.ntiles) break;
-    const int64_t pt = tile * GPT + grp;
-    const bool valid = pt < a.n_ref;
+    int64_t pt = tile * GPT + grp;
+    bool valid = pt < a.n_ref;
+    const int64_t slot_in = pt;
+    if (a.list_in) {
+      uint32_t cnt = __hip_atomic_load(a.fix_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (cnt > a.fix_cap) cnt = a.fix_cap;
+      if (tile * GPT >= (int64_t)cnt) break;  // tiles are taken in order: nothing left
+      valid = slot_in < (int64_t)cnt;
+      pt = valid ? (int64_t)a.list_in[slot_in] : 0;
+    }
     const int64_t fine = a.n0 + pt;
 
     // ---- gather this lane's rows: neighbours first, fine point at row k
@@ -506,7 +517,13 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
           if (lane == il) wv[is] = wi;
         }
       });
-      if (valid) {
+      if (valid && a.list_in) {
+        // FP64 recomputation of a listed point: FP64 weights and sigma at its list slot
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          if (P * r + lane < k) a.w64[slot_in * k + P * r + lane] = (double)wv[r];
+        if (owner) a.sig64[slot_in] = (double)sqrt(skk);
+      } else if (valid) {
 #pragma unroll
         for (int r = 0; r < R; ++r)
           if (P * r + lane < k) a.wts[pt * k + P * r + lane] = wv[r];
@@ -683,25 +700,7 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
   }
 }
 
-// Second phase of the two-phase generate: the FP64 recomputation of the listed points, one
-// wavefront per point, fully parallel (the weights do not depend on any value).
-template <typename T, int D, bool FAST>
-__global__ __launch_bounds__(256) void fix_list_kernel(RefineArgs<T> a) {
-  extern __shared__ __attribute__((aligned(16))) char smem_raw[];
-  if ((a.meta64->log_exact != 0) != FAST) return;
-  // odd row stride (in doubles): the lanes' 64-bit row accesses then fall on distinct banks
-  const int ld = (a.k + 2) | 1;
-  double* slab = reinterpret_cast<double*>(smem_raw) + (threadIdx.x >> 6) * (a.k + 1) * ld;
-  uint32_t count = __hip_atomic_load(a.fix_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (count > a.fix_cap) count = a.fix_cap;
-  for (uint32_t e = blockIdx.x * 4 + (threadIdx.x >> 6); e < count; e += gridDim.x * 4) {
-    const int64_t pt = a.fix_list[e];
-    const double sg = fix64<T, D, FAST>(a, pt, true, slab, ld, a.w64 + (int64_t)e * a.k);
-    if ((threadIdx.x & 63) == 0) a.sig64[e] = sg;
-  }
-}
-
-// Third phase: the dataflow over the stored weights, P lanes per point, lane j % P gathers
+// Last phase of the two-phase generate: the dataflow over the stored weights, P lanes per point, lane j % P gathers
 // neighbour j.  A point waits until all its neighbours' values are published (sentinel =
 // unpublished; a cached read first, the coherent re-read only while it is still the
 // sentinel), then publishes v = [v0] + w . u + sigma xi (FP64 weights and sum for the

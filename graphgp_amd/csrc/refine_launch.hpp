@@ -27,7 +27,7 @@ int launch_variant(const RefineArgs<T>& a, hipStream_t s, size_t lds_bytes, int 
 template <typename T, int KK, int P, int D>
 int launch_config(RefineArgs<T> a, hipStream_t s, size_t lds_bytes, int grid) {
   constexpr int64_t gpt = 256 / P;
-  a.ntiles = (a.n_ref + gpt - 1) / gpt;
+  a.ntiles = ((a.list_in ? (int64_t)a.fix_cap : a.n_ref) + gpt - 1) / gpt;
   if (a.ntiles == 0) return kOk;
   if ((int64_t)grid > a.ntiles) grid = (int)a.ntiles;
   // the FP64 recomputation needs one KK x (KK+1) FP64 slab per wavefront and KK FP64
