@@ -265,11 +265,24 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--sample", type=int, default=2000, help="points checked against the oracle")
     ap.add_argument("--serve-requests", type=int, default=200, help="requests of the serving run (0: skip)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy 1 GiB device-to-device (the reference dispatch for the FETCH_SIZE / "
+                         "WRITE_SIZE counter scale) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); every box in this setup has one GPU",
               file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        import torch
+
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "device copy", "bytes_read": 1 << 30, "bytes_written": 1 << 30}))
+        return None
     if args.workload == "differentiation":
         if args.n == 16_777_216:
             args.n = 4_194_304
@@ -333,7 +346,7 @@ def main():
                         ("fp32", max(2, args.steps // 2))):
         out, wall, kms = timed(prec, steps, args.warmup)
         results[prec] = dict(out=out, ms=wall * 1e3, kernel_ms=kms, pts_per_s=n_ref / wall)
-        log(f"{prec}: {wall * 1e3:.2f} ms/step, refine kernel {kms:.2f} ms, {n_ref / wall / 1e9:.3f} G pts/s")
+        log(f"{prec}: {wall * 1e3:.2f} ms/step, dominant refine kernel {kms:.2f} ms, {n_ref / wall / 1e9:.3f} G pts/s")
 
     # FP64 path checked point by point against the oracle (given its neighbours' FP64 values)
     v64 = results["fp64"]["out"].cpu().numpy()
@@ -365,7 +378,18 @@ def main():
         errs[prec]["local_xi_rms"] = float(np.sqrt(np.nanmean(dx ** 2)))
     log(f"fp64 vs oracle (sampled, local): {loc_err:.3e}; errors vs fp64: {errs}")
 
-    # roofline of the FP32 refinement kernel (VALU bound: no MFMA in this kernel)
+    # roofline of the dominant kernel of the FP32-increment generate, its weights phase
+    # (VALU bound: no MFMA in this kernel); the library's events bracket exactly that launch.
+    # HBM traffic per launch comes from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    # of the same kernel and workload, scaled by a device copy of known size.
+    traffic = None
+    try:
+        with open(os.path.join(ROOT, "profiles", "refine_pmc_16m.md")) as fh:
+            for ln in fh:
+                if ln.startswith("traffic_bytes_per_launch(fp32_increment weights kernel):"):
+                    traffic = float(ln.split(":")[1])
+    except OSError:
+        pass
     r32 = results["fp32_increment"]
     kernel_s = r32["kernel_ms"] * 1e-3
     flops = algorithmic_flops(k, d) * n_ref
@@ -379,8 +403,9 @@ def main():
         "peak": PEAK_FP32_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(tflops / PEAK_FP32_TFLOPS, 4),
-        "traffic": None,
-        "kernel": "refine_kernel<float,32,4,3> (fp32_increment)",
+        "traffic": traffic if n == 16_777_216 and k == 30 else None,
+        "traffic_source": "profiles/refine_pmc_16m.md (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, copy-calibrated)",
+        "kernel": "refine_kernel<float,32,4,3> weights phase of the fp32_increment generate",
         "kernel_ms": round(r32["kernel_ms"], 4),
         "flop_per_point": algorithmic_flops(k, d),
         "bytes_per_point": algorithmic_bytes(k, d, 4, 4),

@@ -180,7 +180,8 @@ int ggp_refine_counters(const void* workspace, uint32_t* host_counters, void* st
 
 /* ---- profiling hook (no reference counterpart): when both are non-NULL hipEvent_t,
  * every refinement flow on this host thread records start_event just before and
- * stop_event just after its refinement kernel, on the call's stream.  Pass NULLs to
+ * stop_event just after its refinement kernel, on the call's stream (for generate in
+ * GGP_PREC_FP32_INC: just after the first, dominant kernel of its three).  Pass NULLs to
  * switch off. */
 int ggp_profile_events(void* start_event, void* stop_event);
 
