@@ -114,6 +114,17 @@ def ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else None
 
 
+def to_host(t):
+    """A small device tensor copied to the host on the current stream, after waiting for
+    that stream.  The copy lands in pinned memory: the runtime stages copies into pageable
+    memory through a buffer that all host threads share, so one could wait on another
+    thread's unrelated work."""
+    h = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+    h.copy_(t, non_blocking=True)
+    torch.cuda.current_stream().synchronize()
+    return h
+
+
 def workspace(nbytes: int):
     return torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device())
 

@@ -167,9 +167,7 @@ int ggp_compute_depths(const int32_t* neighbors, int64_t n_ref, int k, int64_t n
 
 int ggp_compute_depths_status(void* workspace, uint32_t* host_status, void* stream) {
   uint32_t* ctr = static_cast<uint32_t*>(workspace);
-  GGP_HIP(hipMemcpyAsync(host_status, ctr + 1, 4, hipMemcpyDeviceToHost, S(stream)));
-  GGP_HIP(hipStreamSynchronize(S(stream)));
-  return kOk;
+  return read_to_host(host_status, ctr + 1, 4, S(stream));
 }
 
 size_t ggp_order_by_depth_workspace(int64_t n) { return order_by_depth_workspace(n); }
@@ -225,8 +223,7 @@ int ggp_build_graph(const void* points, int64_t n, int d, int dtype, int64_t n0,
                      g.dmax);
   GGP_LAUNCH_CHECK();
   int32_t mx = 0;
-  GGP_HIP(hipMemcpyAsync(&mx, g.dmax, 4, hipMemcpyDeviceToHost, s));
-  GGP_HIP(hipStreamSynchronize(s));
+  GGP_TRY(read_to_host(&mx, g.dmax, 4, s));
   GGP_TRY(order_by_depth(g.tpts, n, d, dtype, g.idx, g.nbrs, n - n0, k, g.depths, mx, points_out, indices_out,
                          neighbors_out, depths_out, g.sub, g.sub_bytes, s));
   if (max_depth_host) *max_depth_host = mx;
@@ -398,9 +395,7 @@ int ggp_cov_matrix(const void* points_a, int64_t na, const void* points_b, int64
 }
 
 int ggp_refine_counters(const void* workspace, uint32_t* host_counters, void* stream) {
-  GGP_HIP(hipMemcpyAsync(host_counters, workspace, 16, hipMemcpyDeviceToHost, S(stream)));
-  GGP_HIP(hipStreamSynchronize(S(stream)));
-  return kOk;
+  return read_to_host(host_counters, workspace, 16, S(stream));
 }
 
 int ggp_profile_events(void* start_event, void* stop_event) {

@@ -157,6 +157,12 @@ __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, int64_t n);
 int fill_u32(void* p, uint32_t v, int64_t n, hipStream_t s);
 int fill_u64(void* p, uint64_t v, int64_t n, hipStream_t s);
 
+// Copy a few bytes (at most 4096) of device memory to the host and wait for them.  The
+// copy goes through a pinned buffer of the calling host thread: the runtime stages copies
+// into pageable memory through a buffer that all threads share, so such a copy can wait
+// on another thread's unrelated work.
+int read_to_host(void* dst, const void* src, size_t bytes, hipStream_t s);
+
 inline int dtype_size(int dtype) { return dtype == 1 ? 8 : 4; }  // 0 = float32, 1 = float64
 
 // CU budget of the calling host thread for persistent (work-queue) launches; 0 = all CUs.
@@ -172,6 +178,12 @@ inline int persistent_grid(const void* kernel, int grid, size_t lds_bytes) {
     per_cu = 1;
   const int cap = budget * per_cu;
   return grid < cap ? grid : cap;
+}
+
+// Grid of a grid-stride launch, capped under the calling thread's CU budget like a
+// persistent launch (long grid-stride kernels of a large job then leave room elsewhere).
+inline unsigned budget_grid(const void* kernel, unsigned grid) {
+  return (unsigned)persistent_grid(kernel, (int)(grid < (1u << 30) ? grid : (1u << 30)), 0);
 }
 
 }  // namespace ggp

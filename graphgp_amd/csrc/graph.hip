@@ -66,20 +66,21 @@ constexpr int kCtrWords = 8;
 __global__ void kahn_init_kernel(const int32_t* __restrict__ nbrs, int64_t n_ref, int k, int64_t n0,
                                  uint32_t* __restrict__ remaining, uint32_t* __restrict__ cnt,
                                  uint32_t* __restrict__ queue, int32_t* __restrict__ depths, uint32_t* ctr) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool bad = false;
-  if (r < n_ref) {
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n_ref; r += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = n0 + r;
     uint32_t rem = 0;
+    bool row_bad = false;
     for (int j = 0; j < k; ++j) {
       const int32_t nb = nbrs[r * k + j];
       if (nb < 0 || (int64_t)nb >= i) {
-        bad = true;
+        row_bad = true;
         break;
       }
       if (nb >= n0) ++rem;
     }
-    if (!bad) {
+    bad = bad || row_bad;
+    if (!row_bad) {
       for (int j = 0; j < k; ++j) {
         const int32_t nb = nbrs[r * k + j];
         if (nb >= n0) atomicAdd(&cnt[nb - n0], 1u);
@@ -115,30 +116,86 @@ __global__ void kahn_start_kernel(uint32_t* ctr) {
 }
 
 // One round: every child of a frontier point loses one unfinished neighbour; a child that
-// reaches zero gets depth d + 1 and joins the next frontier.  The last workgroup to finish
-// moves [head, end) to the points queued in this round.
+// reaches zero gets depth d + 1 and joins the next frontier.  Groups of G lanes take one
+// frontier point each (64 / G points per wavefront and pass) and split its children, so a
+// round costs a few atomic round trips rather than one per child (G >= k fits the average
+// point in one step).  The next frontier is gathered without a hot counter: a wavefront
+// keeps the points it frees in LDS and reserves queue space for them in batches of more
+// than 64, and what is left at the end goes out with one reservation per workgroup.  The
+// last workgroup to finish moves [head, end) to the points queued in this round.
+template <int G>
 __global__ __launch_bounds__(256) void kahn_round_kernel(const int64_t* __restrict__ rowptr,
                                                          const uint32_t* __restrict__ child, int64_t n0,
                                                          uint32_t* __restrict__ remaining, uint32_t* queue,
                                                          int32_t* __restrict__ depths, uint32_t* ctr) {
-  const uint32_t head = __hip_atomic_load(&ctr[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const uint32_t end = __hip_atomic_load(&ctr[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const int32_t d = (int32_t)__hip_atomic_load(&ctr[5], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t q = head + blockIdx.x * blockDim.x + threadIdx.x; q < end; q += stride) {
-    const uint32_t j = queue[q];
-    const int64_t p1 = rowptr[j + 1];
-    for (int64_t p = rowptr[j]; p < p1; ++p) {
-      const uint32_t i = child[p];
-      if (atomicSub(&remaining[i], 1u) == 1u) {
-        depths[n0 + i] = d + 1;
-        queue[atomicAdd(&ctr[4], 1u)] = i;
+  constexpr int PPW = 64 / G;
+  __shared__ uint32_t buf[4][128];
+  __shared__ uint32_t wcnt[4], wbase[4];
+  // The frontier window and depth were set by the previous launch and change only after
+  // every workgroup of this one has read them (the last workgroup moves them), so plain
+  // loads see them.
+  const uint32_t head = ctr[2];
+  const uint32_t end = ctr[3];
+  const int32_t d = (int32_t)ctr[5];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int grp = lane / G, sub = lane % G;
+  const uint64_t below = (uint64_t(1) << lane) - 1;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + w);
+  const uint32_t nwaves = gridDim.x * 4;
+  uint32_t cnt = 0;  // points in buf[w], the same in every lane
+  for (uint32_t qb = head + wave * PPW; qb < end; qb += nwaves * PPW) {
+    const uint32_t q = qb + grp;
+    int64_t p = 0, p1 = 0;
+    if (q < end) {
+      const uint32_t j = queue[q];
+      p = rowptr[j] + sub;
+      p1 = rowptr[j + 1];
+    }
+    while (__ballot(p < p1) != 0) {
+      uint32_t i = 0;
+      bool freed = false;
+      if (p < p1) {
+        i = child[p];
+        freed = atomicSub(&remaining[i], 1u) == 1u;
       }
+      const uint64_t m = __ballot(freed);
+      if (m != 0) {
+        if (freed) {
+          depths[n0 + i] = d + 1;
+          buf[w][cnt + (uint32_t)__popcll(m & below)] = i;
+        }
+        cnt += (uint32_t)__popcll(m);
+        if (cnt > 64) {  // cnt <= 128: at most 64 before this step, plus at most 64
+          __builtin_amdgcn_wave_barrier();
+          uint32_t base = 0;
+          if (lane == 0) base = atomicAdd(&ctr[4], cnt);
+          base = __shfl(base, 0, 64);
+          for (uint32_t e = lane; e < cnt; e += 64) queue[base + e] = buf[w][e];
+          __builtin_amdgcn_wave_barrier();
+          cnt = 0;
+        }
+      }
+      p += G;
     }
   }
+  if (lane == 0) wcnt[w] = cnt;
   __syncthreads();
   if (threadIdx.x == 0) {
-    __threadfence();
+    const uint32_t total = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+    const uint32_t base = total ? atomicAdd(&ctr[4], total) : 0u;
+    wbase[0] = base;
+    wbase[1] = base + wcnt[0];
+    wbase[2] = base + wcnt[0] + wcnt[1];
+    wbase[3] = base + wcnt[0] + wcnt[1] + wcnt[2];
+  }
+  __syncthreads();
+  for (uint32_t e = lane; e < cnt; e += 64) queue[wbase[w] + e] = buf[w][e];
+  // The tail is only touched by returning atomics, all of which have returned before the
+  // barrier, and the depths and queue entries are read by the next launch only; so the
+  // finish count needs no fence (a device-scope fence per workgroup would write back the
+  // L2 each time).
+  __syncthreads();
+  if (threadIdx.x == 0) {
     if (atomicAdd(&ctr[6], 1u) == gridDim.x - 1) {
       const uint32_t tail = __hip_atomic_load(&ctr[4], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(&ctr[2], end, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -147,6 +204,14 @@ __global__ __launch_bounds__(256) void kahn_round_kernel(const int64_t* __restri
       __hip_atomic_store(&ctr[6], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
+}
+
+int launch_round(const void* kernel, unsigned grid, hipStream_t s, const int64_t* rowptr, const uint32_t* child,
+                 int64_t n0, uint32_t* remaining, uint32_t* queue, int32_t* depths, uint32_t* ctr) {
+  void* args[] = {(void*)&rowptr, (void*)&child, (void*)&n0, (void*)&remaining,
+                  (void*)&queue,  (void*)&depths, (void*)&ctr};
+  GGP_HIP(hipLaunchKernel(kernel, dim3(grid), dim3(256), args, 0, s));
+  return kOk;
 }
 
 // ---- exclusive scan of n uint32 counts into n + 1 int64 offsets (three phases)
@@ -327,15 +392,12 @@ int depths_by_sweeps(const int32_t* nbrs, int64_t n_ref, int k, int64_t n0, int3
     hipLaunchKernelGGL(depths_sweep_kernel, dim3(grid_for(n_ref, kDepThreads)), dim3(kDepThreads), 0, s, nbrs, n_ref,
                        k, n0, depths, ctr);
     GGP_LAUNCH_CHECK();
-    GGP_HIP(hipMemcpyAsync(&status, ctr, 4, hipMemcpyDeviceToHost, s));
-    GGP_HIP(hipStreamSynchronize(s));
+    GGP_TRY(read_to_host(&status, ctr, 4, s));
     if (status & 1u) break;     // malformed graph: report through the status word
     if (!(status & 2u)) break;  // fixed point
   }
   uint32_t final_status = (status & 1u) ? 1u : 0u;
-  GGP_HIP(hipMemcpyAsync(ctr + 1, &final_status, 4, hipMemcpyHostToDevice, s));
-  GGP_HIP(hipStreamSynchronize(s));
-  return kOk;
+  return fill_u32(ctr + 1, final_status, 1, s);
 }
 
 }  // namespace
@@ -369,17 +431,14 @@ int compute_depths(const int32_t* nbrs, int64_t n_ref, int k, int64_t n0, int32_
   if (!kahn) return depths_by_sweeps(nbrs, n_ref, k, n0, depths, b.ctr, s);
 
   GGP_TRY(fill_u32(b.cnt, 0u, n_ref, s));
-  hipLaunchKernelGGL(kahn_init_kernel, dim3(grid_for(n_ref, 256)), dim3(256), 0, s, nbrs, n_ref, k, n0, b.remaining,
+  hipLaunchKernelGGL(kahn_init_kernel, dim3(budget_grid((const void*)kahn_init_kernel, grid_capped(n_ref, 256))),
+                     dim3(256), 0, s, nbrs, n_ref, k, n0, b.remaining,
                      b.cnt, b.queue, depths, b.ctr);
   GGP_LAUNCH_CHECK();
   uint32_t bad = 0;
-  GGP_HIP(hipMemcpyAsync(&bad, b.ctr, 4, hipMemcpyDeviceToHost, s));
-  GGP_HIP(hipStreamSynchronize(s));
+  GGP_TRY(read_to_host(&bad, b.ctr, 4, s));
   if (bad) {
-    const uint32_t one = 1;
-    GGP_HIP(hipMemcpyAsync(b.ctr + 1, &one, 4, hipMemcpyHostToDevice, s));
-    GGP_HIP(hipStreamSynchronize(s));
-    return kOk;
+    return fill_u32(b.ctr + 1, 1u, 1, s);
   }
   const int64_t tiles = (n_ref + kScanTile - 1) / kScanTile;
   hipLaunchKernelGGL(scan_tile_sum_kernel, dim3((unsigned)tiles), dim3(256), 0, s, b.cnt, n_ref, b.sums);
@@ -387,32 +446,35 @@ int compute_depths(const int32_t* nbrs, int64_t n_ref, int k, int64_t n0, int32_
   hipLaunchKernelGGL(scan_tile_kernel, dim3((unsigned)tiles), dim3(256), 0, s, b.cnt, n_ref, b.sums, b.rowptr);
   GGP_LAUNCH_CHECK();
   GGP_TRY(fill_u32(b.cnt, 0u, n_ref, s));
-  hipLaunchKernelGGL(kahn_fill_kernel, dim3(grid_capped(n_ref * k, 256)), dim3(256), 0, s, nbrs, n_ref, k, n0,
+  hipLaunchKernelGGL(kahn_fill_kernel, dim3(budget_grid((const void*)kahn_fill_kernel, grid_capped(n_ref * k, 256))),
+                     dim3(256), 0, s, nbrs, n_ref, k, n0,
                      b.rowptr, b.cnt, b.child);
   hipLaunchKernelGGL(kahn_start_kernel, dim3(1), dim3(1), 0, s, b.ctr);
   GGP_LAUNCH_CHECK();
-  const unsigned grid = 512;
+  // G lanes per frontier point with G >= k where possible (a point has k children on
+  // average).  A frontier holds about n_ref / levels points and a few hundred levels is
+  // typical; up to 2048 workgroups (the device's resident wavefronts) share it.
+  const unsigned grid = (unsigned)std::min<int64_t>(std::max<int64_t>(n_ref / 1024, 256), 2048);
+  const void* round_kernel = k <= 8    ? (const void*)kahn_round_kernel<8>
+                             : k <= 16 ? (const void*)kahn_round_kernel<16>
+                             : k <= 32 ? (const void*)kahn_round_kernel<32>
+                                       : (const void*)kahn_round_kernel<64>;
   uint32_t hdr[3] = {0, 0, 0};
   for (int64_t round = 0; round <= n_ref + 1; round += 32) {
     for (int q = 0; q < 32; ++q)
-      hipLaunchKernelGGL(kahn_round_kernel, dim3(grid), dim3(256), 0, s, b.rowptr, b.child, n0, b.remaining,
-                         b.queue, depths, b.ctr);
+      GGP_TRY(launch_round(round_kernel, grid, s, b.rowptr, b.child, n0, b.remaining, b.queue, depths, b.ctr));
     GGP_LAUNCH_CHECK();
-    GGP_HIP(hipMemcpyAsync(hdr, b.ctr + 2, 12, hipMemcpyDeviceToHost, s));
-    GGP_HIP(hipStreamSynchronize(s));
+    GGP_TRY(read_to_host(hdr, b.ctr + 2, 12, s));
     if (hdr[0] == hdr[1]) break;  // empty frontier
   }
   // every refined point must have left a frontier (always true for a graph in topological order)
   const uint32_t final_status = hdr[2] == (uint32_t)n_ref ? 0u : 1u;
-  GGP_HIP(hipMemcpyAsync(b.ctr + 1, &final_status, 4, hipMemcpyHostToDevice, s));
-  GGP_HIP(hipStreamSynchronize(s));
-  return kOk;
+  return fill_u32(b.ctr + 1, final_status, 1, s);
 }
 
 int compute_depths_status(void* ws, uint32_t* host_err, hipStream_t s) {
   uint32_t* ctr = static_cast<uint32_t*>(ws);
-  GGP_HIP(hipMemcpyAsync(host_err, ctr + 1, 4, hipMemcpyDeviceToHost, s));
-  GGP_HIP(hipStreamSynchronize(s));
+  GGP_TRY(read_to_host(host_err, ctr + 1, 4, s));
   return kOk;
 }
 
@@ -460,7 +522,9 @@ int order_by_depth(const void* pts, int64_t n, int d, int dtype, const int32_t* 
                        idx, depths, pay, n, (double*)pts_o, idx_o, dep_o, inv);
   GGP_LAUNCH_CHECK();
   if (n_ref > 0) {
-    hipLaunchKernelGGL(order_neighbors_kernel, dim3(grid_capped(n_ref * k, 256)), dim3(256), 0, s, nbrs, k, pay, inv,
+    hipLaunchKernelGGL(order_neighbors_kernel,
+                       dim3(budget_grid((const void*)order_neighbors_kernel, grid_capped(n_ref * k, 256))), dim3(256),
+                       0, s, nbrs, k, pay, inv,
                        n0, n_ref, nbrs_o);
     GGP_LAUNCH_CHECK();
   }

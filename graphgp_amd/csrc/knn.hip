@@ -113,13 +113,12 @@ __device__ inline bool pair_greater(T da, int32_t ia, T db, int32_t ib) {
   return da > db || (da == db && ia > ib);
 }
 
+// Query t: the k nearest preceding points of point qi, by the reference's traversal.
 template <typename T, int D, int KMAX>
-__global__ __launch_bounds__(256) void knn_kernel(const T* __restrict__ pts, const int32_t* __restrict__ split,
-                                                  int d, const int32_t* __restrict__ qidx,
-                                                  const int32_t* __restrict__ midx, int64_t q0, int64_t nq, int k,
-                                                  int32_t* __restrict__ out, int64_t n, const T* __restrict__ box) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nq) return;
+__device__ __forceinline__ void knn_query(const T* __restrict__ pts, const int32_t* __restrict__ split, int d,
+                                          const int32_t* __restrict__ qidx, const int32_t* __restrict__ midx,
+                                          int64_t q0, int64_t t, int k, int32_t* __restrict__ out, int64_t n,
+                                          const T* __restrict__ box) {
   const int64_t qi = qidx ? (int64_t)qidx[t] : q0 + t;
   const int64_t max_index = min(midx ? (int64_t)midx[t] : qi, n);
   Query<T, D> q;
@@ -198,6 +197,32 @@ __global__ __launch_bounds__(256) void knn_kernel(const T* __restrict__ pts, con
     if (j < k) o[j] = si[j];
 }
 
+// One lane per query.  Plain: workgroup b takes queries [256 b, 256 b + 256).  TICKET (a
+// launch capped below the query count, so that it leaves room for other work on the
+// device): workgroups take blocks of 256 queries from the ticket until none are left.
+template <typename T, int D, int KMAX, bool TICKET>
+__global__ __launch_bounds__(256) void knn_kernel(const T* __restrict__ pts, const int32_t* __restrict__ split,
+                                                  int d, const int32_t* __restrict__ qidx,
+                                                  const int32_t* __restrict__ midx, int64_t q0, int64_t nq, int k,
+                                                  int32_t* __restrict__ out, int64_t n, const T* __restrict__ box,
+                                                  uint32_t* ticket) {
+  if constexpr (!TICKET) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nq) knn_query<T, D, KMAX>(pts, split, d, qidx, midx, q0, t, k, out, n, box);
+  } else {
+    __shared__ uint32_t s_blk;
+    for (;;) {
+      __syncthreads();
+      if (threadIdx.x == 0) s_blk = atomicAdd(ticket, 1u);
+      __syncthreads();
+      const uint32_t blk = s_blk;
+      if ((int64_t)blk * 256 >= nq) break;
+      const int64_t t = (int64_t)blk * 256 + threadIdx.x;
+      if (t < nq) knn_query<T, D, KMAX>(pts, split, d, qidx, midx, q0, t, k, out, n, box);
+    }
+  }
+}
+
 // Generic k: slots in a per-query workspace region (k*(sizeof(T)+4) bytes per query).
 template <typename T>
 __global__ __launch_bounds__(256) void knn_generic_kernel(const T* __restrict__ pts,
@@ -267,19 +292,31 @@ __global__ __launch_bounds__(256) void knn_generic_kernel(const T* __restrict__ 
 }
 
 template <typename T, int D, int KMAX>
-void launch_knn(const T* pts, const int32_t* split, int d, const int32_t* qidx, const int32_t* midx, int64_t q0,
-                int64_t nq, int k, int32_t* out, int64_t n, const T* box, hipStream_t s) {
-  hipLaunchKernelGGL((knn_kernel<T, D, KMAX>), dim3(grid_for(nq, 256)), dim3(256), 0, s, pts, split, d, qidx, midx,
-                     q0, nq, k, out, n, box);
+int launch_knn(const T* pts, const int32_t* split, int d, const int32_t* qidx, const int32_t* midx, int64_t q0,
+               int64_t nq, int k, int32_t* out, int64_t n, const T* box, uint32_t* ticket, hipStream_t s) {
+  const unsigned blocks = grid_for(nq, 256);
+  if (cu_budget() > 0) {  // a long kernel: keep it within the calling thread's CU budget
+    const int g =
+        persistent_grid((const void*)knn_kernel<T, D, KMAX, true>, (int)std::min<unsigned>(blocks, 1u << 30), 0);
+    if ((unsigned)g < blocks) {
+      GGP_TRY(fill_u32(ticket, 0u, 1, s));
+      hipLaunchKernelGGL((knn_kernel<T, D, KMAX, true>), dim3((unsigned)g), dim3(256), 0, s, pts, split, d, qidx,
+                         midx, q0, nq, k, out, n, box, ticket);
+      return kOk;
+    }
+  }
+  hipLaunchKernelGGL((knn_kernel<T, D, KMAX, false>), dim3(blocks), dim3(256), 0, s, pts, split, d, qidx, midx, q0,
+                     nq, k, out, n, box, (uint32_t*)nullptr);
+  return kOk;
 }
 
 template <typename T, int D>
-void dispatch_k(const T* pts, const int32_t* split, int d, const int32_t* qidx, const int32_t* midx, int64_t q0,
-                int64_t nq, int k, int32_t* out, int64_t n, const T* box, hipStream_t s) {
-  if (k <= 8) launch_knn<T, D, 8>(pts, split, d, qidx, midx, q0, nq, k, out, n, box, s);
-  else if (k <= 16) launch_knn<T, D, 16>(pts, split, d, qidx, midx, q0, nq, k, out, n, box, s);
-  else if (k <= 32) launch_knn<T, D, 32>(pts, split, d, qidx, midx, q0, nq, k, out, n, box, s);
-  else launch_knn<T, D, 64>(pts, split, d, qidx, midx, q0, nq, k, out, n, box, s);
+int dispatch_k(const T* pts, const int32_t* split, int d, const int32_t* qidx, const int32_t* midx, int64_t q0,
+               int64_t nq, int k, int32_t* out, int64_t n, const T* box, uint32_t* ticket, hipStream_t s) {
+  if (k <= 8) return launch_knn<T, D, 8>(pts, split, d, qidx, midx, q0, nq, k, out, n, box, ticket, s);
+  if (k <= 16) return launch_knn<T, D, 16>(pts, split, d, qidx, midx, q0, nq, k, out, n, box, ticket, s);
+  if (k <= 32) return launch_knn<T, D, 32>(pts, split, d, qidx, midx, q0, nq, k, out, n, box, ticket, s);
+  return launch_knn<T, D, 64>(pts, split, d, qidx, midx, q0, nq, k, out, n, box, ticket, s);
 }
 
 template <typename T>
@@ -291,6 +328,7 @@ int knn_impl(const T* pts, const int32_t* split, int64_t n, int d, const int32_t
   if (nq <= 0) return kOk;
   Workspace w(ws, ws_bytes);
   T* box = w.take<T>((size_t)n * 2 * d);
+  uint32_t* ticket = w.take<uint32_t>(64);
   T* wsd = nullptr;
   int32_t* wsi = nullptr;
   if (k > 64) {
@@ -313,13 +351,13 @@ int knn_impl(const T* pts, const int32_t* split, int64_t n, int d, const int32_t
     hipLaunchKernelGGL(knn_generic_kernel<T>, dim3(grid_for(nq, 256)), dim3(256), 0, s, pts, split, d, qidx, midx,
                        q0, nq, k, out, wsd, wsi, n, (const T*)box);
   } else if (d == 1) {
-    dispatch_k<T, 1>(pts, split, d, qidx, midx, q0, nq, k, out, n, box, s);
+    GGP_TRY((dispatch_k<T, 1>(pts, split, d, qidx, midx, q0, nq, k, out, n, box, ticket, s)));
   } else if (d == 2) {
-    dispatch_k<T, 2>(pts, split, d, qidx, midx, q0, nq, k, out, n, box, s);
+    GGP_TRY((dispatch_k<T, 2>(pts, split, d, qidx, midx, q0, nq, k, out, n, box, ticket, s)));
   } else if (d == 3) {
-    dispatch_k<T, 3>(pts, split, d, qidx, midx, q0, nq, k, out, n, box, s);
+    GGP_TRY((dispatch_k<T, 3>(pts, split, d, qidx, midx, q0, nq, k, out, n, box, ticket, s)));
   } else {
-    dispatch_k<T, 8>(pts, split, d, qidx, midx, q0, nq, k, out, n, box, s);
+    GGP_TRY((dispatch_k<T, 8>(pts, split, d, qidx, midx, q0, nq, k, out, n, box, ticket, s)));
   }
   GGP_LAUNCH_CHECK();
   return kOk;
@@ -330,6 +368,7 @@ int knn_impl(const T* pts, const int32_t* split, int64_t n, int d, const int32_t
 size_t query_neighbors_workspace(int64_t n, int d, int64_t nq, int k, int dtype) {
   Workspace w(nullptr, 0);
   w.take<char>((size_t)n * 2 * d * dtype_size(dtype));
+  w.take<uint32_t>(64);
   if (k > 64) {
     w.take<char>((size_t)nq * k * dtype_size(dtype));
     w.take<int32_t>((size_t)nq * k);

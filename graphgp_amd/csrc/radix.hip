@@ -9,6 +9,8 @@
 // The element order inside a wave is (item slot, lane), so ranks follow the input
 // order and the sort is stable.  Used by the k-d tree build (segmented value sort
 // per level) and by order_by_depth.
+#include <cstring>
+
 #include "common.hpp"
 #include "radix.hpp"
 
@@ -243,6 +245,19 @@ int fill_u32(void* p, uint32_t v, int64_t n, hipStream_t s) {
   GGP_LAUNCH_CHECK();
   return kOk;
 }
+int read_to_host(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  thread_local void* pinned = nullptr;  // one small buffer per host thread, kept until exit
+  if (bytes > 4096) {
+    set_error("read_to_host: at most 4096 bytes");
+    return kInvalidArgument;
+  }
+  if (pinned == nullptr) GGP_HIP(hipHostMalloc(&pinned, 4096, hipHostMallocDefault));
+  GGP_HIP(hipMemcpyAsync(pinned, src, bytes, hipMemcpyDeviceToHost, s));
+  GGP_HIP(hipStreamSynchronize(s));
+  std::memcpy(dst, pinned, bytes);
+  return kOk;
+}
+
 int fill_u64(void* p, uint64_t v, int64_t n, hipStream_t s) {
   if (n <= 0) return kOk;
   unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 8192);

@@ -50,7 +50,7 @@ def check_graph(graph: Graph):
     L = _lib.lib()
     _lib.check(L.ggp_check_graph(_lib.ptr(nb), n_ref, nb.shape[1], n, _lib.ptr(off), len(offsets), _lib.ptr(status),
                                  _lib.stream()), "check_graph")
-    s = int(status.item())
+    s = int(_lib.to_host(status)[0])
     assert not (s & 2), "Points are not in topological order"
     assert not (s & 4), "Neighbors must not be in the same batch"
 
@@ -60,7 +60,7 @@ def _offsets(depths_sorted: torch.Tensor, max_depth: int) -> Tuple[int, ...]:
     off = torch.empty(n_off, dtype=torch.int64, device=depths_sorted.device)
     _lib.check(_lib.lib().ggp_depth_offsets(_lib.ptr(depths_sorted), depths_sorted.numel(), n_off, _lib.ptr(off),
                                             _lib.stream()), "offsets")
-    return tuple(int(x) for x in off.cpu().tolist())
+    return tuple(int(x) for x in _lib.to_host(off).tolist())
 
 
 def build_graph(points, *, n0: int, k: int, cuda: bool = False, return_depths: bool = False) -> Graph:
@@ -121,7 +121,7 @@ def order_by_depth(points, indices, neighbors, depths, *, cuda: bool = False):
     n_ref, k = nb.shape
     L = _lib.lib()
     dt = _lib.dtype_code(p)
-    max_depth = int(dep.max().item()) if n else 0
+    max_depth = int(_lib.to_host(dep.max())) if n else 0
     po = torch.empty_like(p)
     io = torch.empty(n, dtype=torch.int32, device=p.device)
     no = torch.empty_like(nb)

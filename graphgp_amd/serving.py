@@ -9,9 +9,10 @@ package's own layer on top of the same kernels.  Its parts:
   flight, its workspace and outputs.  A request is admitted only if its bytes fit after
   evicting idle graphs; otherwise it waits for in-flight requests to finish.
 * ``Server``: continuous batching.  Each pass of the scheduler dispatches every request
-  that has arrived and fits the budget, round-robin over a few HIP streams (default
-  priority), so small requests overlap on the GPU instead of queueing behind one another;
-  a request whose graph is not resident is built first (``build_graph``) and then cached.
+  that has arrived and fits the budget, on three HIP streams (default priority: small
+  work, small builds, the large class), so small requests overlap large ones on the GPU
+  instead of queueing behind them; a request whose graph is not resident is built first
+  (``build_graph``, on a builder thread) and then cached.
   Completion is observed with HIP events; latency = completion - arrival on the GPU clock.
 
 A request's result does not depend on what else is in flight: every kernel reads only its
@@ -189,14 +190,17 @@ def percentile(xs, q):
 class Server:
     """Continuous-batching server over HIP streams (see the module docstring).
 
-    Two size classes (up to ``small_n`` points, and larger), each with a stream for its
-    generate / generate_inv work and a builder (thread plus stream) for graphs that are
-    not resident: four streams, one per hardware queue, so small requests never queue
-    behind large ones on the GPU or behind a large graph build on the host.  The builds
-    synchronise with the host inside ``build_graph``; the ctypes calls release the
-    interpreter lock, so dispatch continues meanwhile.  Requests for a graph under
-    construction wait on it; every other arrived request that fits the HBM budget is
-    dispatched at once, in arrival order, skipping those that do not fit yet.
+    Two size classes (up to ``small_n`` points, and larger), each with a builder thread
+    for graphs that are not resident.  Three streams: small work, small builds, and the
+    large class (its builds and its work).  A process gets four hardware queues (HIP's
+    default), one of them taken by the default stream, and streams beyond that share a
+    queue, whose packets run in order: a fourth stream would put some small work behind
+    large kernels.  So small requests never queue behind large ones on the GPU, nor behind
+    a graph build on the host.  The builds synchronise with the host inside
+    ``build_graph``; the ctypes calls release the interpreter lock, so dispatch continues
+    meanwhile.  Requests for a graph under construction wait on it; every other arrived
+    request that fits the HBM budget is dispatched at once, in arrival order, skipping
+    those that do not fit yet.
     """
 
     def __init__(self, hbm_budget_bytes: int = 200 * GiB, max_inflight: int = 64, small_n: int = 1_000_000,
@@ -214,8 +218,9 @@ class Server:
         self.small_n = small_n
         self.max_inflight = max_inflight
         self.build_fn = build_fn or (lambda pts, n0, k: gp.build_graph(pts, n0=n0, k=k))
-        self.work = [torch.cuda.Stream(), torch.cuda.Stream()]  # [small, large]
-        self.build_streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+        small_work, small_build, large = torch.cuda.Stream(), torch.cuda.Stream(), torch.cuda.Stream()
+        self.work = [small_work, large]  # [small, large]
+        self.build_streams = [small_build, large]
         self.builders = [cf.ThreadPoolExecutor(max_workers=1), cf.ThreadPoolExecutor(max_workers=1)]
         cus = torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count
         # persistent launches of large requests leave a share of every CU's slots free
@@ -235,7 +240,12 @@ class Server:
         torch = self.torch
         stream = self.build_streams[self._cls(req)]
         with torch.cuda.stream(stream):
-            g = self.build_fn(req.points, req.n0, req.k)
+            # the long kernels of a large build (the kNN) stay within the CU budget too
+            prev = self._lib.ggp_set_cu_budget(self.large_cus if self._cls(req) else 0)
+            try:
+                g = self.build_fn(req.points, req.n0, req.k)
+            finally:
+                self._lib.ggp_set_cu_budget(prev)
             ev = torch.cuda.Event()
             ev.record(stream)
         return g, ev

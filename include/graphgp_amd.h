@@ -186,10 +186,10 @@ int ggp_refine_counters(const void* workspace, uint32_t* host_counters, void* st
 int ggp_profile_events(void* start_event, void* stop_event);
 
 /* CU budget (no reference counterpart; for a serving loop that runs small and large
- * requests concurrently): the persistent refinement and adjoint launches issued by the
- * calling host thread use at most `cus` CUs' worth of resident workgroups (0 = all CUs),
- * leaving room for other streams' kernels.  Results do not depend on it.  Returns the
- * previous value. */
+ * requests concurrently): the persistent refinement and adjoint launches, and the kNN
+ * query (then a ticketed persistent launch), issued by the calling host thread use at most
+ * `cus` CUs' worth of resident workgroups (0 = all CUs), leaving room for other streams'
+ * kernels.  Results do not depend on it.  Returns the previous value. */
 int ggp_set_cu_budget(int cus);
 
 #ifdef __cplusplus

@@ -100,6 +100,21 @@ def test_query_neighbors_explicit_indices():
     assert np.array_equal(cpu(nb), O.query_neighbors(op, osd, q, m, k=12))
 
 
+def test_query_preceding_under_cu_budget():
+    # a CU budget turns the kNN launch into a ticketed persistent grid; neighbours unchanged
+    p = clustered_points(60000, d=3, seed=77, dtype=np.float32)
+    op, osd, _ = O.build_tree(p)
+    lib = gp._lib.lib()
+    prev = lib.ggp_set_cu_budget(2)
+    try:
+        nb = gp.query_preceding_neighbors(op, osd, n0=32, k=32)
+        nb8 = gp.query_preceding_neighbors(op, osd, n0=8, k=8)
+    finally:
+        lib.ggp_set_cu_budget(prev)
+    assert np.array_equal(cpu(nb), O.query_preceding_neighbors(op, osd, n0=32, k=32))
+    assert np.array_equal(cpu(nb8), O.query_preceding_neighbors(op, osd, n0=8, k=8))
+
+
 def test_query_n0_less_than_k_raises():
     pts, sd, _ = gp.build_tree(ref_points((100, 2)))
     with pytest.raises(ValueError, match="n0 must be at least k"):
@@ -128,6 +143,19 @@ def test_compute_depths_random_vs_oracle(method):
             nb = np.maximum(nb, 0).astype(np.int32)
         d = cpu(gp.compute_depths(nb, n0=n0, method=method))
         assert np.array_equal(d, O.compute_depths(nb, n0=n0)), (n0, n_ref, k)
+
+
+@pytest.mark.parametrize("method", ["rounds", "sweeps"])
+def test_compute_depths_hub_and_chain(method):
+    # one refined point is a neighbour of every later point (thousands of children), the
+    # second column chains the points, the third is random
+    rng = np.random.default_rng(11)
+    n0, n_ref = 4, 5000
+    i = n0 + np.arange(n_ref)
+    nb = np.stack([np.where(i > n0, n0, 0), np.maximum(i - 1, 0), rng.integers(0, i)], axis=1).astype(np.int32)
+    d = cpu(gp.compute_depths(nb, n0=n0, method=method))
+    assert np.array_equal(d, O.compute_depths(nb, n0=n0))
+    assert d.max() == n_ref
 
 
 def test_order_by_depth_kat():
