@@ -204,7 +204,7 @@ class Server:
     """
 
     def __init__(self, hbm_budget_bytes: int = 200 * GiB, max_inflight: int = 64, small_n: int = 1_000_000,
-                 large_cu_fraction: float = 0.75, build_fn: Optional[Callable] = None):
+                 large_cu_fraction: float = 0.5, build_fn: Optional[Callable] = None):
         import concurrent.futures as cf
 
         import torch
@@ -223,7 +223,10 @@ class Server:
         self.build_streams = [small_build, large]
         self.builders = [cf.ThreadPoolExecutor(max_workers=1), cf.ThreadPoolExecutor(max_workers=1)]
         cus = torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count
-        # persistent launches of large requests leave a share of every CU's slots free
+        # The long launches of large requests (persistent refinement and adjoint, the kNN and
+        # the grid-stride passes of a build) stay within this share of the CUs, leaving
+        # slots free for small requests.  A smaller share lowers the small requests' tail
+        # latency and the throughput of a device full of large work (1.0: no reservation).
         self.large_cus = max(1, int(cus * large_cu_fraction)) if large_cu_fraction < 1 else 0
         self._lib = gp._lib.lib()
 
@@ -466,30 +469,38 @@ def materialize(specs, device="cuda"):
     return reqs
 
 
-def serve_benchmark(n_requests: int = 200, seed: int = 5, load: float = 0.8) -> Dict[str, Any]:
-    """The serving measurement: the synthetic mix served all at once (batch capacity),
-    then with Poisson arrivals at ``load`` x that request rate.  Fresh servers (no resident
-    graphs) for both runs."""
+def serve_benchmark(n_requests: int = 2000, seed: int = 5, load: float = 0.8,
+                    large_cu_fraction: float = 0.5) -> Dict[str, Any]:
+    """The serving measurement: the synthetic mix served all at once (batch mode, the
+    server's capacity), then with Poisson arrivals at ``load`` x that request rate, both
+    with the same server policy; fresh servers (no resident graphs) for every run.  Also
+    the batch throughput with no CU reservation for small requests, so that the cost of
+    the reservation shows."""
     import torch
 
-    specs = synthetic_mix(n_requests, rate_per_s=1e9, seed=seed)
-    reqs = materialize(specs)
-    torch.cuda.synchronize()
-    srv = Server()
-    batch = srv.run(reqs, realtime=False)
-    srv.close()
+    def batch_run(frac):
+        reqs = materialize(synthetic_mix(n_requests, rate_per_s=1e9, seed=seed))
+        torch.cuda.synchronize()
+        srv = Server(large_cu_fraction=frac)
+        out = srv.run(reqs, realtime=False)
+        srv.close()
+        return out
+
+    full = batch_run(1.0)
+    batch = batch_run(large_cu_fraction)
     rate = load * batch["requests"] / batch["wall_s"]
-    del reqs
     reqs = materialize(synthetic_mix(n_requests, rate_per_s=rate, seed=seed))
     torch.cuda.synchronize()
-    srv = Server()
+    srv = Server(large_cu_fraction=large_cu_fraction)
     serve = srv.run(reqs, realtime=True)
     srv.close()
     return {
         "requests": n_requests,
         "arrival_rate_per_s": round(rate, 2),
         "load": load,
+        "large_cu_fraction": large_cu_fraction,
         "batch_points_per_s": round(batch["points_per_s"], 1),
+        "batch_points_per_s_no_reservation": round(full["points_per_s"], 1),
         "serve_points_per_s": round(serve["points_per_s"], 1),
         "serve_vs_batch": round(serve["points_per_s"] / batch["points_per_s"], 4),
         "p50_ms": round(serve["p50_ms"], 3),
@@ -506,7 +517,7 @@ if __name__ == "__main__":
     import json
 
     ap = argparse.ArgumentParser(description="serving benchmark over the synthetic request mix")
-    ap.add_argument("--requests", type=int, default=200)
+    ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--seed", type=int, default=5)
     ap.add_argument("--load", type=float, default=0.8)
     a = ap.parse_args()

@@ -50,6 +50,16 @@ def test_build_tree_vs_oracle_f32_clustered(n, d, seed):
     assert np.array_equal(cpu(pts), op)
 
 
+@pytest.mark.parametrize("n", [3000, 30000])
+def test_build_tree_f64_lattice(n):
+    # float64 keys with many ties, all levels in one workgroup (3000) and the hybrid (30000)
+    rng = np.random.default_rng(n)
+    p = rng.integers(0, 9, (n, 3)).astype(np.float64) * 0.25
+    pts, sd, idx = gp.build_tree(p)
+    op, osd, oidx = O.build_tree(p)
+    assert np.array_equal(cpu(idx), oidx) and np.array_equal(cpu(sd), osd)
+
+
 def test_build_tree_duplicates_and_ties():
     rng = np.random.default_rng(11)
     p = rng.integers(0, 7, (20000, 3)).astype(np.float32)
