@@ -63,6 +63,7 @@ _SIGS = {
     "ggp_profile_events": (_int, [_p, _p]),
     "ggp_refine_counters": (_int, [_p, _u32p, _p]),
     "ggp_set_cu_budget": (_int, [_int]),
+    "ggp_set_yield": (_int, [_p, _int]),
 }
 
 _lib = None

@@ -19,6 +19,20 @@ thread_local int g_cu_budget = 0;
 void set_error(const std::string& msg) { g_error = msg; }
 const char* last_error() { return g_error.c_str(); }
 int& cu_budget() { return g_cu_budget; }
+thread_local YieldCfg g_yield;
+YieldCfg& yield_cfg() { return g_yield; }
+
+int device_cus() {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0, c = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        c <= 0)
+      c = 256;
+    cus = c;
+  }
+  return cus;
+}
 
 namespace {
 
@@ -121,6 +135,20 @@ int ggp_set_cu_budget(int cus) {
   const int prev = ggp::cu_budget();
   ggp::cu_budget() = cus > 0 ? cus : 0;
   return prev;
+}
+int ggp_set_yield(const uint32_t* flag, int keep_cus) {
+  YieldCfg& y = ggp::yield_cfg();
+  if (flag == nullptr) {
+    y = YieldCfg{};
+    return kOk;
+  }
+  GGP_CHECK_ARG(keep_cus >= 1, "set_yield: keep_cus must be at least 1");
+  void* dev = nullptr;
+  GGP_HIP(hipHostGetDevicePointer(&dev, const_cast<uint32_t*>(flag), 0));
+  y.host = flag;
+  y.dev = static_cast<const uint32_t*>(dev);
+  y.keep_cus = keep_cus;
+  return kOk;
 }
 int ggp_abi_version(void) { return 1; }
 

@@ -168,22 +168,62 @@ inline int dtype_size(int dtype) { return dtype == 1 ? 8 : 4; }  // 0 = float32,
 // CU budget of the calling host thread for persistent (work-queue) launches; 0 = all CUs.
 int& cu_budget();
 
+// Yield setting of the calling host thread (ggp_set_yield): while the word at `host`
+// (pinned host memory, `dev` its device address) is nonzero, the workgroups of persistent
+// launches beyond the first keep_cus CUs' worth stop taking work and exit, and grid-stride
+// passes launched meanwhile stay within keep_cus CUs.
+struct YieldCfg {
+  const volatile uint32_t* host = nullptr;
+  const uint32_t* dev = nullptr;
+  int keep_cus = 0;
+};
+YieldCfg& yield_cfg();
+int device_cus();
+
+// Workgroups of `kernel` (256 threads, lds_bytes of LDS) resident per CU.
+inline int blocks_per_cu(const void* kernel, size_t lds_bytes) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, lds_bytes) != hipSuccess || per_cu < 1)
+    per_cu = 1;
+  return per_cu;
+}
+
 // Grid of a persistent launch: at most `grid` workgroups and, under a CU budget, at most
 // budget x (workgroups of `kernel` resident per CU), so the launch leaves room elsewhere.
 inline int persistent_grid(const void* kernel, int grid, size_t lds_bytes) {
   const int budget = cu_budget();
   if (budget <= 0) return grid;
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, lds_bytes) != hipSuccess || per_cu < 1)
-    per_cu = 1;
-  const int cap = budget * per_cu;
+  const int cap = budget * blocks_per_cu(kernel, lds_bytes);
   return grid < cap ? grid : cap;
 }
 
+// Workgroups of a persistent launch of `grid` (spread over the whole device) that never
+// yield: keep_cus CUs' share of them, at least one.
+inline uint32_t yield_keep(int grid) {
+  const YieldCfg& y = yield_cfg();
+  if (y.dev == nullptr) return (uint32_t)grid;
+  const int cus = device_cus();
+  const int64_t keep = ((int64_t)grid * y.keep_cus + cus - 1) / cus;
+  return (uint32_t)(keep < 1 ? 1 : (keep > grid ? grid : keep));
+}
+
+// Whether workgroup blockIdx.x of a persistent launch should stop taking work now.
+__device__ inline bool yield_now(const uint32_t* flag, uint32_t keep) {
+  return flag != nullptr && blockIdx.x >= keep &&
+         __hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0u;
+}
+
 // Grid of a grid-stride launch, capped under the calling thread's CU budget like a
-// persistent launch (long grid-stride kernels of a large job then leave room elsewhere).
+// persistent launch, and to the yield share while the yield word is set (long grid-stride
+// kernels of a large job then leave room elsewhere).
 inline unsigned budget_grid(const void* kernel, unsigned grid) {
-  return (unsigned)persistent_grid(kernel, (int)(grid < (1u << 30) ? grid : (1u << 30)), 0);
+  int g = persistent_grid(kernel, (int)(grid < (1u << 30) ? grid : (1u << 30)), 0);
+  const YieldCfg& y = yield_cfg();
+  if (y.host != nullptr && *y.host != 0u) {
+    const int cap = y.keep_cus * blocks_per_cu(kernel, 0);
+    if (cap >= 1 && g > cap) g = cap;
+  }
+  return (unsigned)g;
 }
 
 }  // namespace ggp

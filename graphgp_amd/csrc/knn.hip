@@ -205,7 +205,7 @@ __global__ __launch_bounds__(256) void knn_kernel(const T* __restrict__ pts, con
                                                   int d, const int32_t* __restrict__ qidx,
                                                   const int32_t* __restrict__ midx, int64_t q0, int64_t nq, int k,
                                                   int32_t* __restrict__ out, int64_t n, const T* __restrict__ box,
-                                                  uint32_t* ticket) {
+                                                  uint32_t* ticket, const uint32_t* yield, uint32_t keep) {
   if constexpr (!TICKET) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < nq) knn_query<T, D, KMAX>(pts, split, d, qidx, midx, q0, t, k, out, n, box);
@@ -213,7 +213,7 @@ __global__ __launch_bounds__(256) void knn_kernel(const T* __restrict__ pts, con
     __shared__ uint32_t s_blk;
     for (;;) {
       __syncthreads();
-      if (threadIdx.x == 0) s_blk = atomicAdd(ticket, 1u);
+      if (threadIdx.x == 0) s_blk = yield_now(yield, keep) ? 0xffffffffu : atomicAdd(ticket, 1u);
       __syncthreads();
       const uint32_t blk = s_blk;
       if ((int64_t)blk * 256 >= nq) break;
@@ -295,18 +295,21 @@ template <typename T, int D, int KMAX>
 int launch_knn(const T* pts, const int32_t* split, int d, const int32_t* qidx, const int32_t* midx, int64_t q0,
                int64_t nq, int k, int32_t* out, int64_t n, const T* box, uint32_t* ticket, hipStream_t s) {
   const unsigned blocks = grid_for(nq, 256);
-  if (cu_budget() > 0) {  // a long kernel: keep it within the calling thread's CU budget
-    const int g =
-        persistent_grid((const void*)knn_kernel<T, D, KMAX, true>, (int)std::min<unsigned>(blocks, 1u << 30), 0);
-    if ((unsigned)g < blocks) {
+  const YieldCfg& y = yield_cfg();
+  if (cu_budget() > 0 || y.dev != nullptr) {
+    // a long kernel: within the calling thread's CU budget, or yielding while asked to
+    const void* kern = (const void*)knn_kernel<T, D, KMAX, true>;
+    const int cus = cu_budget() > 0 ? cu_budget() : device_cus();
+    const int g = (int)std::min<int64_t>(blocks, (int64_t)cus * blocks_per_cu(kern, 0));
+    if ((unsigned)g < blocks || y.dev != nullptr) {
       GGP_TRY(fill_u32(ticket, 0u, 1, s));
       hipLaunchKernelGGL((knn_kernel<T, D, KMAX, true>), dim3((unsigned)g), dim3(256), 0, s, pts, split, d, qidx,
-                         midx, q0, nq, k, out, n, box, ticket);
+                         midx, q0, nq, k, out, n, box, ticket, y.dev, yield_keep(g));
       return kOk;
     }
   }
   hipLaunchKernelGGL((knn_kernel<T, D, KMAX, false>), dim3(blocks), dim3(256), 0, s, pts, split, d, qidx, midx, q0,
-                     nq, k, out, n, box, (uint32_t*)nullptr);
+                     nq, k, out, n, box, (uint32_t*)nullptr, (const uint32_t*)nullptr, 0u);
   return kOk;
 }
 

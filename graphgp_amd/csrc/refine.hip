@@ -347,16 +347,6 @@ double fix_tau() {
   return e ? atof(e) : kDefaultFixTau;
 }
 
-int device_cus() {
-  static int cus = 0;
-  if (cus == 0) {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-      cus = 256;
-  }
-  return cus;
-}
-
 size_t max_lds() {
   static size_t lds = 0;
   if (lds == 0) {
@@ -553,6 +543,8 @@ int run_two_phase(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hipSt
   a.ntiles = (p.n_ref + 256 / P - 1) / (256 / P);
   auto kern = apply_kernel<T, P, RM>;
   const int grid = (int)std::min<int64_t>(a.ntiles, persistent_grid((const void*)kern, cus * 8, 0));
+  a.yield = yield_cfg().dev;
+  a.keep = yield_keep(grid);
   hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, s, a);
   GGP_LAUNCH_CHECK();
   return kOk;

@@ -83,6 +83,9 @@ struct RefineArgs {
   // weights mode over a list: tile slot i is point list_in[i] for i < min(*fix_count,
   // fix_cap); outputs go to w64 / sig64 at slot i
   const uint32_t* list_in;
+  // yield: while *yield is nonzero, workgroups from `keep` on stop taking tiles
+  const uint32_t* yield;
+  uint32_t keep;
 };
 
 // ------------------------------------------------------------------ helpers
@@ -365,7 +368,7 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
   const uint64_t gmask = (P == 64) ? ~0ull : ((1ull << P) - 1ull);
 
   for (;;) {
-    if (threadIdx.x == 0) s_tile = atomicAdd(a.ticket, 1u);
+    if (threadIdx.x == 0) s_tile = yield_now(a.yield, a.keep) ? 0xffffffffu : atomicAdd(a.ticket, 1u);
     __syncthreads();
     const int64_t tile = s_tile;
     __syncthreads();
@@ -717,7 +720,7 @@ __global__ __launch_bounds__(256) void apply_kernel(RefineArgs<T> a) {
   const int gbase = (threadIdx.x & 63) & ~(P - 1);
   const uint64_t gmask = (P == 64) ? ~0ull : ((1ull << P) - 1ull);
   for (;;) {
-    if (threadIdx.x == 0) s_tile = atomicAdd(a.ticket, 1u);
+    if (threadIdx.x == 0) s_tile = yield_now(a.yield, a.keep) ? 0xffffffffu : atomicAdd(a.ticket, 1u);
     __syncthreads();
     const int64_t tile = s_tile;
     __syncthreads();

@@ -17,7 +17,10 @@ int launch_variant(const RefineArgs<T>& a, hipStream_t s, size_t lds_bytes, int 
     }
   }
   grid = persistent_grid((const void*)kern, grid, lds_bytes);
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds_bytes, s, a);
+  RefineArgs<T> b = a;
+  b.yield = yield_cfg().dev;
+  b.keep = yield_keep(grid);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds_bytes, s, b);
   GGP_LAUNCH_CHECK();
   return kOk;
 }

@@ -257,63 +257,37 @@ __global__ __launch_bounds__(kSubThreads) void subtree_kernel(const T* __restric
   uint32_t* nxt = cur + kSubCap;
   uint8_t* spl = reinterpret_cast<uint8_t*>(nxt + kSubCap);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const uint32_t i = blockIdx.x;
   const uint32_t stride0 = 1u << L0;
-  {
-    const Level l0(n, L0);
-    const uint32_t st = l0.start(i), sz = l0.size(i);
-    for (uint32_t t = tid; t < sz; t += kSubThreads) cur[t] = act[st + t];
-  }
-  __syncthreads();
-  for (int L = L0; L < levels; ++L) {
-    const Level lv(n, L);
-    const bool last = L == levels - 1;
-    const uint32_t M = 1u << (L - L0);
-    const uint32_t P = pow2_at_least(lv.q + (lv.r ? 1u : 0u));
-    const int lgP = __builtin_ctz(P);
-    // extents and split dimension (first argmax of max - min, in T), as split_kernel
-    auto split_of = [&](const K* mn, const K* mx) {
-      int best = 0;
-      T bestv = KeyOf<T>::value(mx[0]) - KeyOf<T>::value(mn[0]);
-      for (int j = 1; j < d; ++j) {
-        const T e = KeyOf<T>::value(mx[j]) - KeyOf<T>::value(mn[j]);
-        if (e > bestv) { bestv = e; best = j; }
-      }
-      return best;
-    };
-    if (P >= 64) {
-      for (uint32_t m = wave; m < M; m += kSubThreads / 64) {
-        const uint32_t sg = i + m * stride0, sz = lv.size(sg);
-        K mn[8], mx[8];
-        for (int j = 0; j < d; ++j) { mn[j] = ~K(0); mx[j] = K(0); }
-        for (uint32_t t = lane; t < sz; t += 64) {
-          const T* x = pts + (size_t)cur[(m << lgP) + t] * d;
-          for (int j = 0; j < d; ++j) {
-            const K kk = KeyOf<T>::key(x[j]);
-            mn[j] = kmin_(mn[j], kk);
-            mx[j] = kmax_(mx[j], kk);
-          }
-        }
-        for (int j = 0; j < d; ++j)
-          for (int o = 32; o > 0; o >>= 1) {
-            mn[j] = kmin_(mn[j], (K)__shfl_xor(mn[j], o, 64));
-            mx[j] = kmax_(mx[j], (K)__shfl_xor(mx[j], o, 64));
-          }
-        if (lane == 0) {
-          const int best = sz > 1 ? split_of(mn, mx) : 0;
-          spl[m] = (uint8_t)best;
-          split_dims[lv.n_above + sg] = best;
-        }
-      }
-    } else {
-      for (uint32_t m = tid; m < M; m += kSubThreads) {
-        const uint32_t sg = i + m * stride0, sz = lv.size(sg);
-        if (sz == 0) continue;
+  const Level l0(n, L0);
+  const uint32_t n_seg0 = min(l0.n_level, l0.n_act);
+  for (uint32_t i = blockIdx.x; i < n_seg0; i += gridDim.x) {
+    {
+      const uint32_t st = l0.start(i), sz = l0.size(i);
+      for (uint32_t t = tid; t < sz; t += kSubThreads) cur[t] = act[st + t];
+    }
+    __syncthreads();
+    for (int L = L0; L < levels; ++L) {
+      const Level lv(n, L);
+      const bool last = L == levels - 1;
+      const uint32_t M = 1u << (L - L0);
+      const uint32_t P = pow2_at_least(lv.q + (lv.r ? 1u : 0u));
+      const int lgP = __builtin_ctz(P);
+      // extents and split dimension (first argmax of max - min, in T), as split_kernel
+      auto split_of = [&](const K* mn, const K* mx) {
         int best = 0;
-        if (sz > 1) {
+        T bestv = KeyOf<T>::value(mx[0]) - KeyOf<T>::value(mn[0]);
+        for (int j = 1; j < d; ++j) {
+          const T e = KeyOf<T>::value(mx[j]) - KeyOf<T>::value(mn[j]);
+          if (e > bestv) { bestv = e; best = j; }
+        }
+        return best;
+      };
+      if (P >= 64) {
+        for (uint32_t m = wave; m < M; m += kSubThreads / 64) {
+          const uint32_t sg = i + m * stride0, sz = lv.size(sg);
           K mn[8], mx[8];
           for (int j = 0; j < d; ++j) { mn[j] = ~K(0); mx[j] = K(0); }
-          for (uint32_t t = 0; t < sz; ++t) {
+          for (uint32_t t = lane; t < sz; t += 64) {
             const T* x = pts + (size_t)cur[(m << lgP) + t] * d;
             for (int j = 0; j < d; ++j) {
               const K kk = KeyOf<T>::key(x[j]);
@@ -321,76 +295,104 @@ __global__ __launch_bounds__(kSubThreads) void subtree_kernel(const T* __restric
               mx[j] = kmax_(mx[j], kk);
             }
           }
-          best = split_of(mn, mx);
-        }
-        spl[m] = (uint8_t)best;
-        split_dims[lv.n_above + sg] = best;
-      }
-    }
-    __syncthreads();
-    const uint32_t total = M << lgP;
-    if (P > 1) {
-      // keys: (coordinate key, position in the previous order); empty slot entries last
-      for (uint32_t e = tid; e < total; e += kSubThreads) {
-        const uint32_t m = e >> lgP, t = e & (P - 1);
-        const uint32_t sz = lv.size(i + m * stride0);
-        if (t < sz) {
-          const K kk = KeyOf<T>::key(pts[(size_t)cur[e] * d + spl[m]]);
-          if constexpr (kWide) {
-            skey[e] = (uint64_t)kk;
-            spos[e] = t;
-          } else {
-            skey[e] = ((uint64_t)kk << 32) | t;
+          for (int j = 0; j < d; ++j)
+            for (int o = 32; o > 0; o >>= 1) {
+              mn[j] = kmin_(mn[j], (K)__shfl_xor(mn[j], o, 64));
+              mx[j] = kmax_(mx[j], (K)__shfl_xor(mx[j], o, 64));
+            }
+          if (lane == 0) {
+            const int best = sz > 1 ? split_of(mn, mx) : 0;
+            spl[m] = (uint8_t)best;
+            split_dims[lv.n_above + sg] = best;
           }
-        } else {
-          skey[e] = ~uint64_t(0);
-          if constexpr (kWide) spos[e] = ~0u;
+        }
+      } else {
+        for (uint32_t m = tid; m < M; m += kSubThreads) {
+          const uint32_t sg = i + m * stride0, sz = lv.size(sg);
+          if (sz == 0) continue;
+          int best = 0;
+          if (sz > 1) {
+            K mn[8], mx[8];
+            for (int j = 0; j < d; ++j) { mn[j] = ~K(0); mx[j] = K(0); }
+            for (uint32_t t = 0; t < sz; ++t) {
+              const T* x = pts + (size_t)cur[(m << lgP) + t] * d;
+              for (int j = 0; j < d; ++j) {
+                const K kk = KeyOf<T>::key(x[j]);
+                mn[j] = kmin_(mn[j], kk);
+                mx[j] = kmax_(mx[j], kk);
+              }
+            }
+            best = split_of(mn, mx);
+          }
+          spl[m] = (uint8_t)best;
+          split_dims[lv.n_above + sg] = best;
         }
       }
       __syncthreads();
-      // bitonic sort of every slot of P entries, ascending
-      for (uint32_t k = 2; k <= P; k <<= 1) {
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-          for (uint32_t e = tid; e < total; e += kSubThreads) {
-            const uint32_t x = e ^ j;
-            if (x > e) {
-              const bool asc = ((e & (P - 1)) & k) == 0;
-              const uint64_t a = skey[e], b = skey[x];
-              bool gt;
-              if constexpr (kWide) gt = a > b || (a == b && spos[e] > spos[x]);
-              else gt = a > b;
-              if (gt == asc) {
-                skey[e] = b;
-                skey[x] = a;
-                if constexpr (kWide) {
-                  const uint32_t pa = spos[e];
-                  spos[e] = spos[x];
-                  spos[x] = pa;
+      const uint32_t total = M << lgP;
+      if (P > 1) {
+        // keys: (coordinate key, position in the previous order); empty slot entries last
+        for (uint32_t e = tid; e < total; e += kSubThreads) {
+          const uint32_t m = e >> lgP, t = e & (P - 1);
+          const uint32_t sz = lv.size(i + m * stride0);
+          if (t < sz) {
+            const K kk = KeyOf<T>::key(pts[(size_t)cur[e] * d + spl[m]]);
+            if constexpr (kWide) {
+              skey[e] = (uint64_t)kk;
+              spos[e] = t;
+            } else {
+              skey[e] = ((uint64_t)kk << 32) | t;
+            }
+          } else {
+            skey[e] = ~uint64_t(0);
+            if constexpr (kWide) spos[e] = ~0u;
+          }
+        }
+        __syncthreads();
+        // bitonic sort of every slot of P entries, ascending
+        for (uint32_t k = 2; k <= P; k <<= 1) {
+          for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t e = tid; e < total; e += kSubThreads) {
+              const uint32_t x = e ^ j;
+              if (x > e) {
+                const bool asc = ((e & (P - 1)) & k) == 0;
+                const uint64_t a = skey[e], b = skey[x];
+                bool gt;
+                if constexpr (kWide) gt = a > b || (a == b && spos[e] > spos[x]);
+                else gt = a > b;
+                if (gt == asc) {
+                  skey[e] = b;
+                  skey[x] = a;
+                  if constexpr (kWide) {
+                    const uint32_t pa = spos[e];
+                    spos[e] = spos[x];
+                    spos[x] = pa;
+                  }
                 }
               }
             }
+            __syncthreads();
           }
-          __syncthreads();
         }
       }
+      // median -> node id; lower half -> slot m, upper half -> slot m + M of the next level
+      const uint32_t Pn = last ? 1u : pow2_at_least(Level(n, L + 1).q + (Level(n, L + 1).r ? 1u : 0u));
+      for (uint32_t e = tid; e < total; e += kSubThreads) {
+        const uint32_t m = e >> lgP, t = e & (P - 1);
+        const uint32_t sg = i + m * stride0, sz = lv.size(sg);
+        if (t >= sz) continue;
+        uint32_t pos = t;
+        if (P > 1) pos = kWide ? spos[e] : (uint32_t)(skey[e] & 0xffffffffu);
+        const uint32_t id = cur[(m << lgP) + pos];
+        const uint32_t mid = sz / 2u;
+        if (t == mid) final_idx[lv.n_above + sg] = (int32_t)id;
+        else if (!last) nxt[t < mid ? m * Pn + t : (m + M) * Pn + (t - mid - 1u)] = id;
+      }
+      __syncthreads();
+      uint32_t* sw = cur;
+      cur = nxt;
+      nxt = sw;
     }
-    // median -> node id; lower half -> slot m, upper half -> slot m + M of the next level
-    const uint32_t Pn = last ? 1u : pow2_at_least(Level(n, L + 1).q + (Level(n, L + 1).r ? 1u : 0u));
-    for (uint32_t e = tid; e < total; e += kSubThreads) {
-      const uint32_t m = e >> lgP, t = e & (P - 1);
-      const uint32_t sg = i + m * stride0, sz = lv.size(sg);
-      if (t >= sz) continue;
-      uint32_t pos = t;
-      if (P > 1) pos = kWide ? spos[e] : (uint32_t)(skey[e] & 0xffffffffu);
-      const uint32_t id = cur[(m << lgP) + pos];
-      const uint32_t mid = sz / 2u;
-      if (t == mid) final_idx[lv.n_above + sg] = (int32_t)id;
-      else if (!last) nxt[t < mid ? m * Pn + t : (m + M) * Pn + (t - mid - 1u)] = id;
-    }
-    __syncthreads();
-    uint32_t* sw = cur;
-    cur = nxt;
-    nxt = sw;
   }
 }
 
@@ -530,8 +532,16 @@ static int build_tree_impl(const T* pts, int64_t n64, int d, T* pts_out, int32_t
                                   (int)lds));
       lds_set = true;
     }
-    hipLaunchKernelGGL(subtree_kernel<T>, dim3(std::min(l0.n_level, l0.n_act)), dim3(kSubThreads), lds, s, pts, d, n,
-                       L0, levels, act, split_dims, indices);
+    // one workgroup per segment, fewer under a CU budget or while asked to yield
+    unsigned grid = std::min(l0.n_level, l0.n_act);
+    if (cu_budget() > 0 || yield_cfg().host != nullptr) {
+      const int per_cu = blocks_per_cu((const void*)subtree_kernel<T>, lds);
+      int cus = cu_budget() > 0 ? cu_budget() : device_cus();
+      if (yield_cfg().host != nullptr && *yield_cfg().host != 0u) cus = std::min(cus, yield_cfg().keep_cus);
+      grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(grid, (int64_t)cus * per_cu));
+    }
+    hipLaunchKernelGGL(subtree_kernel<T>, dim3(grid), dim3(kSubThreads), lds, s, pts, d, n, L0, levels, act,
+                       split_dims, indices);
     GGP_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL(gather_points_kernel<T>, dim3(grid_capped(n64 * d, 256)), dim3(256), 0, s, pts, d, indices,

@@ -204,7 +204,7 @@ class Server:
     """
 
     def __init__(self, hbm_budget_bytes: int = 200 * GiB, max_inflight: int = 64, small_n: int = 1_000_000,
-                 large_cu_fraction: float = 0.5, build_fn: Optional[Callable] = None):
+                 large_cu_fraction: float = 0.35, build_fn: Optional[Callable] = None):
         import concurrent.futures as cf
 
         import torch
@@ -223,11 +223,16 @@ class Server:
         self.build_streams = [small_build, large]
         self.builders = [cf.ThreadPoolExecutor(max_workers=1), cf.ThreadPoolExecutor(max_workers=1)]
         cus = torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count
-        # The long launches of large requests (persistent refinement and adjoint, the kNN and
-        # the grid-stride passes of a build) stay within this share of the CUs, leaving
-        # slots free for small requests.  A smaller share lowers the small requests' tail
-        # latency and the throughput of a device full of large work (1.0: no reservation).
+        # While small requests are waiting or running, the long launches of large requests
+        # (persistent refinement phases and the kNN, which yield: their extra workgroups
+        # stop taking work; grid-stride passes of a build, sized at launch) keep this share
+        # of the CUs and leave the rest to the small requests; otherwise they use the whole
+        # device.  The flag the kernels read is a word of pinned host memory that the
+        # scheduler sets.  A smaller share lowers the small requests' tail latency (1.0: no
+        # reservation).
         self.large_cus = max(1, int(cus * large_cu_fraction)) if large_cu_fraction < 1 else 0
+        self._yield = torch.zeros(1, dtype=torch.int32, pin_memory=True)
+        self._yield_np = self._yield.numpy()
         self._lib = gp._lib.lib()
 
     def close(self):
@@ -239,16 +244,20 @@ class Server:
     def _cls(self, req: Request) -> int:
         return 0 if req.n <= self.small_n else 1
 
+    def _reserve(self, req: Request):
+        """Launches of a large request on this thread yield to small requests."""
+        if self._cls(req) and self.large_cus:
+            self.gp._lib.check(self._lib.ggp_set_yield(self._yield.data_ptr(), self.large_cus), "set_yield")
+
     def _build(self, req: Request):
         torch = self.torch
         stream = self.build_streams[self._cls(req)]
         with torch.cuda.stream(stream):
-            # the long kernels of a large build (the kNN) stay within the CU budget too
-            prev = self._lib.ggp_set_cu_budget(self.large_cus if self._cls(req) else 0)
+            self._reserve(req)
             try:
                 g = self.build_fn(req.points, req.n0, req.k)
             finally:
-                self._lib.ggp_set_cu_budget(prev)
+                self._lib.ggp_set_yield(None, 0)
             ev = torch.cuda.Event()
             ev.record(stream)
         return g, ev
@@ -261,7 +270,7 @@ class Server:
                 stream.wait_event(after)
             req._start_ev = torch.cuda.Event(enable_timing=True)
             req._start_ev.record(stream)
-            prev = self._lib.ggp_set_cu_budget(self.large_cus if self._cls(req) else 0)
+            self._reserve(req)
             try:
                 if req.op == "generate":
                     out = gp.generate(g, req.covariance, req.data, precision=req.precision)
@@ -270,7 +279,7 @@ class Server:
                 else:
                     raise ValueError(f"unknown op {req.op!r}")
             finally:
-                self._lib.ggp_set_cu_budget(prev)
+                self._lib.ggp_set_yield(None, 0)
             ev = torch.cuda.Event(enable_timing=True)
             ev.record(stream)
         return out, ev
@@ -331,10 +340,17 @@ class Server:
                 progressed = True
             return progressed
 
+        small = self.small_n
+
+        def small_active():
+            return (any(r.n <= small for r in waiting) or any(r.n <= small for r, _, _ in inflight)
+                    or any(b[1][0].n <= small for b in building.values()))
+
         while pending or waiting or inflight or building:
             now = time.perf_counter() - t0
             while pending and (not realtime or pending[0].arrival <= now):
                 waiting.append(pending.popleft())
+            self._yield_np[0] = 1 if small_active() else 0
             progressed = finish_builds()
             still = []
             for req in waiting:
@@ -362,12 +378,14 @@ class Server:
                 still.append(req)
             waiting = still
             retire(block=False)
+            self._yield_np[0] = 1 if small_active() else 0
             if not progressed:
                 if inflight or building:
                     time.sleep(1e-4)  # poll completions, builds and arrivals
                 elif pending and realtime:
                     time.sleep(max(0.0, min(0.002, pending[0].arrival - (time.perf_counter() - t0))))
         retire(block=True)
+        self._yield_np[0] = 0
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         lat = [r.latency_s for r in done]
@@ -469,8 +487,8 @@ def materialize(specs, device="cuda"):
     return reqs
 
 
-def serve_benchmark(n_requests: int = 2000, seed: int = 5, load: float = 0.8,
-                    large_cu_fraction: float = 0.5) -> Dict[str, Any]:
+def serve_benchmark(n_requests: int = 2000, seed: int = 5, load: float = 0.85,
+                    large_cu_fraction: float = 0.35) -> Dict[str, Any]:
     """The serving measurement: the synthetic mix served all at once (batch mode, the
     server's capacity), then with Poisson arrivals at ``load`` x that request rate, both
     with the same server policy; fresh servers (no resident graphs) for every run.  Also
@@ -519,6 +537,6 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser(description="serving benchmark over the synthetic request mix")
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--seed", type=int, default=5)
-    ap.add_argument("--load", type=float, default=0.8)
+    ap.add_argument("--load", type=float, default=0.85)
     a = ap.parse_args()
     print(json.dumps(serve_benchmark(a.requests, a.seed, a.load)))

@@ -192,6 +192,16 @@ int ggp_profile_events(void* start_event, void* stop_event);
  * kernels.  Results do not depend on it.  Returns the previous value. */
 int ggp_set_cu_budget(int cus);
 
+/* Yield (no reference counterpart; for a serving loop that reserves CUs for small requests
+ * only while there are some): the persistent launches issued by the calling host thread
+ * (refinement phases, and the kNN query, then ticketed) may fill the device, but while the
+ * 32-bit word at `flag` is nonzero their workgroups beyond the first `keep_cus` CUs' worth
+ * stop taking work and exit, leaving those slots to other streams; grid-stride passes
+ * launched while it is nonzero stay within `keep_cus` CUs.  `flag` is pinned host memory
+ * (hipHostMalloc) that the host writes; NULL switches the setting off.  Results do not
+ * depend on it. */
+int ggp_set_yield(const uint32_t* flag, int keep_cus);
+
 #ifdef __cplusplus
 }
 #endif

@@ -78,3 +78,57 @@ def test_result_alone_equals_result_in_mix():
         assert alone.dtype == r.result.dtype == torch.float32
         assert torch.equal(alone.view(torch.int32), r.result.view(torch.int32)), (r.rid, r.op, r.n)
         assert not torch.isnan(r.result).any(), (r.rid, r.op, r.n)
+
+
+@pytest.mark.gpu
+def test_yielding_launches_give_the_same_results():
+    # with the yield word set, workgroups beyond the kept CU share stop taking work at once:
+    # generate (three precisions), the inverse and the kNN must not change
+    import torch
+
+    import graphgp_amd as gp
+    from graphgp_amd.serving import make_points, make_table
+
+    lib = gp._lib.lib()
+    flag = torch.ones(1, dtype=torch.int32, pin_memory=True)
+    pts = make_points(150_000, 3, True, 3)
+    tab = make_table("matern32", 0.05, 1000)
+    xi = torch.randn(150_000, device="cuda", generator=torch.Generator(device="cuda").manual_seed(4))
+    ref = {}
+    g = gp.build_graph(pts, n0=1000, k=16)
+    for prec in ["fp64", "fp32", "fp32_increment"]:
+        ref[prec] = (gp.generate(g, tab, xi, precision=prec), gp.generate_inv(g, tab, xi, precision=prec))
+    for value in (1, 0):
+        flag[0] = value
+        gp._lib.check(lib.ggp_set_yield(flag.data_ptr(), 1), "set_yield")
+        try:
+            gy = gp.build_graph(pts, n0=1000, k=16)
+            for prec in ["fp64", "fp32", "fp32_increment"]:
+                v = gp.generate(gy, tab, xi, precision=prec)
+                u = gp.generate_inv(gy, tab, xi, precision=prec)
+                assert torch.equal(v, ref[prec][0]) and torch.equal(u, ref[prec][1]), (value, prec)
+        finally:
+            lib.ggp_set_yield(None, 0)
+        assert torch.equal(gy.neighbors, g.neighbors) and torch.equal(gy.points, g.points)
+        assert gy.offsets == g.offsets
+
+
+@pytest.mark.gpu
+def test_large_and_small_requests_alone_equal_in_mix():
+    # a mix with large requests (their launches yield while small requests run)
+    import torch
+
+    import graphgp_amd as gp
+    from graphgp_amd.serving import Server, materialize
+
+    specs = synthetic_mix(16, rate_per_s=1e4, seed=12, n_min=2e4, n_max=3e6)
+    reqs = materialize(specs)
+    assert any(r.n > 1_000_000 for r in reqs) and any(r.n <= 1_000_000 for r in reqs)
+    srv = Server(max_inflight=8)
+    srv.run(reqs, realtime=False)
+    srv.close()
+    for r in reqs:
+        g = gp.build_graph(r.points, n0=r.n0, k=r.k)
+        fn = gp.generate if r.op == "generate" else gp.generate_inv
+        alone = fn(g, r.covariance, r.data, precision=r.precision)
+        assert torch.equal(alone.view(torch.int32), r.result.view(torch.int32)), (r.rid, r.op, r.n)
