@@ -120,3 +120,36 @@ def test_vjp_clustered_k30():
     ogx, ogv = O.generate_vjp(og, ocov, xi, w)
     assert np.allclose(cpu(gx), ogx, rtol=1e-6, atol=1e-7 * np.abs(ogx).max())
     assert np.linalg.norm(cpu(gv) - ogv) <= 1e-6 * np.linalg.norm(ogv)
+
+
+@pytest.mark.parametrize("kind", ["rbf", "matern32"])
+def test_grad_wrt_kernel_hyperparameters(kind):
+    # The table producers run on the device with autograd, so the adjoint kernel's table
+    # gradient reaches the hyperparameters.  Values scale as sqrt(variance) exactly (the
+    # jitter multiplies the variance too): dL/dvariance = L / (2 variance).  The length
+    # scale by central differences of the forward pass.
+    rng = np.random.default_rng(21)
+    n = 6000
+    p = rng.uniform(0, 1, (n, 3)).astype(np.float32)
+    g = gp.build_graph(p, n0=200, k=12)
+    xi = torch.as_tensor(rng.standard_normal(n), device="cuda")
+    w = torch.as_tensor(rng.standard_normal(n), device="cuda")
+    var0, len0 = 1.3, 0.15
+
+    def table(v, s):
+        if kind == "rbf":
+            return gp.extras.rbf_kernel(variance=v, scale=s, r_min=1e-5, r_max=2, n_bins=1000, jitter=1e-5)
+        return gp.extras.matern_kernel(p=1, variance=v, cutoff=s, r_min=1e-5, r_max=2, n_bins=1000, jitter=1e-5)
+
+    def loss(v, s):
+        return (w * gp.generate(g, table(v, s), xi, precision="fp64")).sum()
+
+    v = torch.tensor(var0, dtype=torch.float64, device="cuda", requires_grad=True)
+    s = torch.tensor(len0, dtype=torch.float64, device="cuda", requires_grad=True)
+    L = loss(v, s)
+    gv, gs = torch.autograd.grad(L, [v, s])
+    assert abs(gv.item() - L.item() / (2 * var0)) <= 1e-9 * abs(L.item() / (2 * var0))
+    h = 1e-6 * len0
+    with torch.no_grad():
+        fd = (loss(var0, len0 + h) - loss(var0, len0 - h)).item() / (2 * h)
+    assert abs(gs.item() - fd) <= 1e-5 * max(abs(fd), 1.0), (gs.item(), fd)
