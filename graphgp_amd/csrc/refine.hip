@@ -539,6 +539,8 @@ int run_two_phase(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hipSt
   const int cus = device_cus();
   GGP_TRY(fill_u32(b.ctr, 0u, 1, s));  // fresh ticket for the dataflow
   a.mode = kGenerate;
+  a.sched = p.sched;
+  a.nbv = p.sched_nbrs;
   constexpr int P = 4, RM = 10;  // k <= 40 for the register configurations
   a.ntiles = (p.n_ref + 256 / P - 1) / (256 / P);
   auto kern = apply_kernel<T, P, RM>;

@@ -86,6 +86,11 @@ struct RefineArgs {
   // yield: while *yield is nonzero, workgroups from `keep` on stop taking tiles
   const uint32_t* yield;
   uint32_t keep;
+  // dataflow over stored weights in schedule order (or null): slot q takes refined point
+  // sched[q], reads its neighbours' values at the slots nbv[q, .] and stores its value at
+  // slot n0 + q
+  const int32_t* sched;
+  const int32_t* nbv;
 };
 
 // ------------------------------------------------------------------ helpers
@@ -725,9 +730,11 @@ __global__ __launch_bounds__(256) void apply_kernel(RefineArgs<T> a) {
     const int64_t tile = s_tile;
     __syncthreads();
     if (tile >= a.ntiles) break;
-    const int64_t pt = tile * GPT + grp;
-    const bool valid = pt < a.n_ref;
+    const int64_t q = tile * GPT + grp;  // slot: the refined point's index, or its schedule slot
+    const bool valid = q < a.n_ref;
+    const int64_t pt = (a.sched != nullptr && valid) ? (int64_t)a.sched[q] : q;
     const int64_t fine = a.n0 + pt;
+    const int64_t store = a.n0 + q;
     const int32_t slot = valid ? a.fixslot[pt] : -1;
     int32_t nb[RM];
     double wd[RM];
@@ -738,7 +745,7 @@ __global__ __launch_bounds__(256) void apply_kernel(RefineArgs<T> a) {
       nb[r] = 0;
       wd[r] = 0.0;
       if (valid && j < k) {
-        nb[r] = a.nbrs[pt * k + j];
+        nb[r] = a.nbv != nullptr ? a.nbv[q * k + j] : a.nbrs[pt * k + j];
         wd[r] = slot >= 0 ? a.w64[(int64_t)slot * k + j] : (double)a.wts[pt * k + j];
         pend |= 1u << r;
       }
@@ -797,7 +804,7 @@ __global__ __launch_bounds__(256) void apply_kernel(RefineArgs<T> a) {
               atomicOr(&a.flags[0], 1u);
               v = __builtin_nan("");
             }
-            __hip_atomic_store(&a.vals[fine], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&a.vals[store], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (a.out) {
               if (a.out_f64) ((double*)a.out)[dst] = (double)v;
               else ((float*)a.out)[dst] = (float)v;
@@ -811,7 +818,7 @@ __global__ __launch_bounds__(256) void apply_kernel(RefineArgs<T> a) {
         if (!done) {
           atomicOr(&a.flags[1], 1u);
           atomicOr(&a.flags[0], 1u);
-          if (owner) __hip_atomic_store(&a.vals[fine], (T)__builtin_nan(""), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (owner) __hip_atomic_store(&a.vals[store], (T)__builtin_nan(""), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         break;
       }

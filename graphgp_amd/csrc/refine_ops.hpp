@@ -18,6 +18,9 @@ struct RefineProblem {
   int nb;
   int increment;  // FP32 increment formulation
   int with_seed;  // 1: generate/generate_inv/generate_logdet (dense seed block); 0: raw refine*
+  // generate in the FP32 increment mode: the dataflow's schedule (ggp_graph_schedule) or null
+  const int32_t* sched = nullptr;
+  const int32_t* sched_nbrs = nullptr;
 };
 
 struct RefineIO {

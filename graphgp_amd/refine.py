@@ -73,20 +73,56 @@ def generate(graph: Graph, covariance, xi, *, cuda: bool = False, fast_jit: bool
         from .autograd import GenerateFunction
 
         return GenerateFunction.apply(x, vals, pts, nb, perm, bins, prec)
-    return _generate_raw(pts, nb, perm, bins, vals, x, prec, return_counters)
+    sched = _schedule(graph, pts, nb) if prec == "fp32_increment" else None
+    return _generate_raw(pts, nb, perm, bins, vals, x, prec, return_counters, sched)
 
 
-def _generate_raw(pts, nb, perm, bins, vals, x, prec, return_counters=False):
+# Graphs from this many refined points on get a dataflow schedule (ggp_graph_schedule):
+# ordering every batch by space makes a tile's neighbour values share cache lines.  It
+# costs one sort and an (n - n0, k) int32 array per graph, kept on the Graph.
+SCHEDULE_MIN_REFINED = 1 << 20
+
+
+def _schedule(graph: Graph, pts, nb):
+    """The graph's dataflow schedule, computed on first use and kept on the graph (its
+    arrays are treated as immutable, like the reference's JAX arrays); None for small
+    graphs or more than 65535 batches."""
+    n, d = pts.shape
+    n_ref, k = nb.shape
+    if n_ref < SCHEDULE_MIN_REFINED or len(graph.offsets) > 65536:
+        return None
+    key = (pts.data_ptr(), nb.data_ptr(), tuple(graph.offsets))
+    cached = getattr(graph, "_ggp_schedule", None)
+    if cached is not None and cached[0] == key:
+        return cached[1]
+    n0 = n - n_ref
+    off = torch.tensor(graph.offsets, dtype=torch.int64, device=pts.device)
+    order = torch.empty(n_ref, dtype=torch.int32, device=pts.device)
+    nbv = torch.empty((n_ref, k), dtype=torch.int32, device=pts.device)
+    L = _lib.lib()
+    ws = _lib.workspace(L.ggp_graph_schedule_workspace(n, n0))
+    _lib.check(L.ggp_graph_schedule(_lib.ptr(pts), _lib.dtype_code(pts), n, d, _lib.ptr(nb), n0, k, _lib.ptr(off),
+                                    off.numel(), _lib.ptr(order), _lib.ptr(nbv), _lib.ptr(ws), ws.numel(),
+                                    _lib.stream()), "graph schedule")
+    graph._ggp_schedule = (key, (order, nbv))
+    return order, nbv
+
+
+def _generate_raw(pts, nb, perm, bins, vals, x, prec, return_counters=False, sched=None):
     n, d = pts.shape
     n_ref, k = nb.shape
     n0 = n - n_ref
     out = torch.empty(n, dtype=_out_dtype(prec), device=pts.device)
     L = _lib.lib()
     ws = _lib.workspace(L.ggp_refine_workspace(n, d, n0, k, bins.numel(), _lib.PREC[prec], 1))
-    _lib.check(L.ggp_generate(_lib.ptr(pts), _lib.dtype_code(pts), n, d, _lib.ptr(nb), n0, k, _lib.ptr(perm),
-                              _lib.ptr(bins), _lib.ptr(vals), bins.numel(), _lib.ptr(x), _lib.dtype_code(x),
-                              _lib.ptr(out), _lib.dtype_code(out), _lib.PREC[prec], _lib.ptr(ws), ws.numel(),
-                              _lib.stream()), "generate")
+    args = (_lib.ptr(pts), _lib.dtype_code(pts), n, d, _lib.ptr(nb), n0, k, _lib.ptr(perm), _lib.ptr(bins),
+            _lib.ptr(vals), bins.numel(), _lib.ptr(x), _lib.dtype_code(x), _lib.ptr(out), _lib.dtype_code(out),
+            _lib.PREC[prec])
+    if sched is not None:
+        _lib.check(L.ggp_generate_sched(*args, _lib.ptr(sched[0]), _lib.ptr(sched[1]), _lib.ptr(ws), ws.numel(),
+                                        _lib.stream()), "generate")
+    else:
+        _lib.check(L.ggp_generate(*args, _lib.ptr(ws), ws.numel(), _lib.stream()), "generate")
     return (out, _counters(ws)) if return_counters else out
 
 

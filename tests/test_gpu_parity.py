@@ -371,3 +371,33 @@ def test_dense_seed_block_vs_oracle(n):
     assert np.allclose(v, ov, rtol=1e-9, atol=1e-11)
     assert np.allclose(cpu(gp.generate_dense_inv(p, cov, ov)), xi, rtol=1e-7, atol=1e-8)
     assert np.isclose(float(gp.generate_dense_logdet(p, cov)), O.generate_dense_logdet(p, ocov), rtol=1e-11)
+
+
+def test_scheduled_generate_equals_unscheduled(monkeypatch):
+    # the dataflow schedule changes only the order of evaluation and storage: bit-identical
+    # values; the schedule is a permutation that keeps the batch order
+    import sys
+
+    R = sys.modules["graphgp_amd.refine"]  # the module (the package exports a function of that name)
+    p = clustered_points(200_000, d=3, seed=31)
+    g = gp.build_graph(p, n0=1000, k=30)
+    cov = gp.extras.matern_kernel(p=1, variance=1.0, cutoff=0.05, r_min=1e-6, r_max=2, n_bins=2001, jitter=1e-5)
+    xi = torch.randn(200_000, device="cuda", generator=torch.Generator(device="cuda").manual_seed(3))
+    a = gp.generate(g, cov, xi, precision="fp32_increment")
+    assert getattr(g, "_ggp_schedule", None) is None
+    monkeypatch.setattr(R, "SCHEDULE_MIN_REFINED", 0)
+    b = gp.generate(g, cov, xi, precision="fp32_increment")
+    c = gp.generate(g, cov, xi, precision="fp32_increment")  # cached schedule
+    assert torch.equal(a.view(torch.int32), b.view(torch.int32))
+    assert torch.equal(a.view(torch.int32), c.view(torch.int32))
+    order, nbv = g._ggp_schedule[1]
+    o = cpu(order).astype(np.int64)
+    n0 = g.offsets[0]
+    assert np.array_equal(np.sort(o), np.arange(len(o)))
+    batch = np.searchsorted(np.asarray(g.offsets), n0 + o, side="right") - 1
+    assert np.all(np.diff(batch) >= 0)
+    slot = np.empty_like(o)
+    slot[o] = np.arange(len(o))
+    nb = cpu(g.neighbors).astype(np.int64)[o]
+    expect = np.where(nb < n0, nb, n0 + slot[np.maximum(nb - n0, 0)])
+    assert np.array_equal(cpu(nbv), expect)
