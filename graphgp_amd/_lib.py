@@ -45,9 +45,9 @@ _SIGS = {
     "ggp_generate": (_int, [_p, _int, _i64, _int, _p, _i64, _int, _p, _p, _p, _int, _p, _int, _p, _int, _int, _p,
                             _c_size, _p]),
     "ggp_generate_sched": (_int, [_p, _int, _i64, _int, _p, _i64, _int, _p, _p, _p, _int, _p, _int, _p, _int, _int,
-                                  _p, _p, _p, _c_size, _p]),
+                                  _p, _p, _p, _p, _c_size, _p]),
     "ggp_graph_schedule_workspace": (_c_size, [_i64, _i64]),
-    "ggp_graph_schedule": (_int, [_p, _int, _i64, _int, _p, _i64, _int, _p, _i64, _p, _p, _p, _c_size, _p]),
+    "ggp_graph_schedule": (_int, [_p, _int, _i64, _int, _p, _i64, _int, _p, _i64, _p, _p, _p, _p, _c_size, _p]),
     "ggp_generate_vjp_workspace": (_c_size, [_i64, _int, _i64, _int, _int, _int]),
     "ggp_generate_vjp": (_int, [_p, _int, _i64, _int, _p, _i64, _int, _p, _p, _p, _int, _p, _int, _p, _int, _p,
                                 _int, _p, _int, _p, _int, _p, _c_size, _p]),

@@ -292,17 +292,19 @@ int ggp_generate(const void* points, int pdtype, int64_t n, int d, const int32_t
 int ggp_generate_sched(const void* points, int pdtype, int64_t n, int d, const int32_t* neighbors, int64_t n0, int k,
                        const int32_t* indices, const double* cov_bins, const double* cov_vals, int n_bins,
                        const void* xi, int xi_dtype, void* values, int values_dtype, int precision,
-                       const int32_t* schedule, const int32_t* schedule_neighbors, void* workspace,
-                       size_t workspace_bytes, void* stream) {
+                       const int32_t* schedule, const int32_t* schedule_neighbors, const void* schedule_points,
+                       void* workspace, size_t workspace_bytes, void* stream) {
   GGP_TRY(check_refine_args(n, n0, k, precision, pdtype));
   GGP_TRY(check_dtype(xi_dtype, "generate xi"));
   GGP_CHECK_ARG(values_dtype == (precision == GGP_PREC_FP64 ? GGP_F64 : GGP_F32),
                 "generate: values dtype must be the compute dtype (FP64 for GGP_PREC_FP64, FP32 otherwise)");
-  GGP_CHECK_ARG((schedule == nullptr) == (schedule_neighbors == nullptr),
-                "generate: schedule and schedule_neighbors go together");
+  GGP_CHECK_ARG((schedule == nullptr) == (schedule_neighbors == nullptr) &&
+                    (schedule == nullptr) == (schedule_points == nullptr),
+                "generate: schedule, schedule_neighbors and schedule_points go together");
   RefineProblem p = problem(points, pdtype, n, d, neighbors, n0, k, indices, cov_bins, cov_vals, n_bins, precision, 1);
   p.sched = schedule;
   p.sched_nbrs = schedule_neighbors;
+  p.sched_pts = schedule_points;
   RefineIO io{};
   io.xi = xi;
   io.xi_f64 = xi_dtype == GGP_F64;
@@ -315,10 +317,10 @@ size_t ggp_graph_schedule_workspace(int64_t n, int64_t n0) { return graph_schedu
 
 int ggp_graph_schedule(const void* points, int pdtype, int64_t n, int d, const int32_t* neighbors, int64_t n0, int k,
                        const int64_t* offsets, int64_t n_offsets, int32_t* schedule, int32_t* schedule_neighbors,
-                       void* workspace, size_t workspace_bytes, void* stream) {
+                       void* schedule_points, void* workspace, size_t workspace_bytes, void* stream) {
   GGP_TRY(check_dtype(pdtype, "graph_schedule points"));
   return graph_schedule(points, pdtype == GGP_F64 ? 1 : 0, n, d, neighbors, n0, k, offsets, n_offsets, schedule,
-                        schedule_neighbors, workspace, workspace_bytes, S(stream));
+                        schedule_neighbors, schedule_points, workspace, workspace_bytes, S(stream));
 }
 
 size_t ggp_generate_vjp_workspace(int64_t n, int d, int64_t n0, int k, int n_bins, int precision) {

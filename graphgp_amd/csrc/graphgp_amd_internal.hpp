@@ -13,8 +13,8 @@ int build_tree(const void* pts, int64_t n, int d, int dtype, void* pts_out, int3
 size_t query_neighbors_workspace(int64_t n, int d, int64_t nq, int k, int dtype);
 size_t graph_schedule_workspace(int64_t n_ref);
 int graph_schedule(const void* pts, int dtype, int64_t n, int d, const int32_t* nbrs, int64_t n0, int k,
-                   const int64_t* offsets, int64_t n_off, int32_t* sched, int32_t* nbv, void* ws, size_t ws_bytes,
-                   hipStream_t s);
+                   const int64_t* offsets, int64_t n_off, int32_t* sched, int32_t* nbv, void* pts_sched, void* ws,
+                   size_t ws_bytes, hipStream_t s);
 int query_neighbors(const void* pts, const int32_t* split, int64_t n, int d, int dtype, const int32_t* qidx,
                     const int32_t* midx, int64_t q0, int64_t nq, int k, int32_t* out, void* ws, size_t ws_bytes,
                     hipStream_t s);

@@ -505,8 +505,15 @@ template <typename T>
 int run_two_phase(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hipStream_t s) {
   if (g_prof_start) GGP_HIP(hipEventRecord(g_prof_start, s));
   setup_args<T>(a, p, b);  // the later phases launch with this copy of the arguments
+  // with a schedule, the weights and FP64 phases work in slot space (points in slot order,
+  // neighbours as value slots), so their weights, sigma and list are slot-indexed
+  const void* w_pts = p.sched ? p.sched_pts : p.pts;
+  const int32_t* w_nbrs = p.sched ? p.sched_nbrs : p.nbrs;
+  RefineProblem pw = p;  // run_kernel_inner sets the kernel's inputs from its problem
+  pw.pts = w_pts;
+  pw.nbrs = w_nbrs;
   a.mode = kWeights;
-  GGP_TRY(run_kernel_inner<T>(a, p, b, s));
+  GGP_TRY(run_kernel_inner<T>(a, pw, b, s));
   // the profiling events bracket the dominant kernel: the weights phase
   if (g_prof_stop) GGP_HIP(hipEventRecord(g_prof_stop, s));
   // FP64 recomputation of the listed points with the FP64 register kernel (weights mode over
@@ -514,10 +521,10 @@ int run_two_phase(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hipSt
   {
     GGP_TRY(fill_u32(b.ctr, 0u, 1, s));  // fresh ticket
     RefineArgs<double> d{};
-    d.pts = p.pts;
+    d.pts = w_pts;
     d.pts_f64 = p.pts_f64;
     d.d = p.d;
-    d.nbrs = p.nbrs;
+    d.nbrs = w_nbrs;
     d.k = p.k;
     d.n0 = p.n - p.n_ref;
     d.n_ref = p.n_ref;
@@ -539,6 +546,8 @@ int run_two_phase(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hipSt
   const int cus = device_cus();
   GGP_TRY(fill_u32(b.ctr, 0u, 1, s));  // fresh ticket for the dataflow
   a.mode = kGenerate;
+  a.pts = p.pts;
+  a.nbrs = p.nbrs;
   a.sched = p.sched;
   a.nbv = p.sched_nbrs;
   constexpr int P = 4, RM = 10;  // k <= 40 for the register configurations

@@ -87,8 +87,9 @@ struct RefineArgs {
   const uint32_t* yield;
   uint32_t keep;
   // dataflow over stored weights in schedule order (or null): slot q takes refined point
-  // sched[q], reads its neighbours' values at the slots nbv[q, .] and stores its value at
-  // slot n0 + q
+  // sched[q], reads its weights and sigma at slot q (the weights phase then ran in slot
+  // space), its neighbours' values at the slots nbv[q, .], and stores its value at slot
+  // n0 + q
   const int32_t* sched;
   const int32_t* nbv;
 };
@@ -733,9 +734,9 @@ __global__ __launch_bounds__(256) void apply_kernel(RefineArgs<T> a) {
     const int64_t q = tile * GPT + grp;  // slot: the refined point's index, or its schedule slot
     const bool valid = q < a.n_ref;
     const int64_t pt = (a.sched != nullptr && valid) ? (int64_t)a.sched[q] : q;
-    const int64_t fine = a.n0 + pt;
-    const int64_t store = a.n0 + q;
-    const int32_t slot = valid ? a.fixslot[pt] : -1;
+    const int64_t fine = a.n0 + pt;  // graph position: xi and output through perm
+    const int64_t store = a.n0 + q;  // value slot; weights, sigma and fixslot are slot-indexed
+    const int32_t slot = valid ? a.fixslot[q] : -1;
     int32_t nb[RM];
     double wd[RM];
     uint32_t pend = 0;
@@ -746,7 +747,7 @@ __global__ __launch_bounds__(256) void apply_kernel(RefineArgs<T> a) {
       wd[r] = 0.0;
       if (valid && j < k) {
         nb[r] = a.nbv != nullptr ? a.nbv[q * k + j] : a.nbrs[pt * k + j];
-        wd[r] = slot >= 0 ? a.w64[(int64_t)slot * k + j] : (double)a.wts[pt * k + j];
+        wd[r] = slot >= 0 ? a.w64[(int64_t)slot * k + j] : (double)a.wts[q * k + j];
         pend |= 1u << r;
       }
     }
@@ -756,7 +757,7 @@ __global__ __launch_bounds__(256) void apply_kernel(RefineArgs<T> a) {
     if (valid && owner) {
       dst = a.perm ? (int64_t)a.perm[fine] : fine;
       xi = load_as<T>(a.xi, a.xi_f64, dst - a.io_base);
-      sg = slot >= 0 ? (T)0 : a.sig[pt];
+      sg = slot >= 0 ? (T)0 : a.sig[q];
     }
     T u[RM];
 #pragma unroll

@@ -21,6 +21,7 @@ struct RefineProblem {
   // generate in the FP32 increment mode: the dataflow's schedule (ggp_graph_schedule) or null
   const int32_t* sched = nullptr;
   const int32_t* sched_nbrs = nullptr;
+  const void* sched_pts = nullptr;  // points in slot order (dtype of pts)
 };
 
 struct RefineIO {

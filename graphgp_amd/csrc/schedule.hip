@@ -13,6 +13,8 @@
 //   sched[q]   graph index (0-based over the refined points) of schedule slot q
 //   nbv[q, j]  neighbour j of sched[q] as a value slot: seeds keep their graph position,
 //              refined neighbour r moves to n0 + (slot of r)
+//   pts_sched  the points in slot order (optional): with nbv as neighbours, the refinement
+//              kernel then works in slot space, its tiles' coordinate gathers local too
 #include "common.hpp"
 #include "radix.hpp"
 #include "graphgp_amd_internal.hpp"
@@ -101,6 +103,18 @@ __global__ void sched_nbrs_kernel(const int32_t* __restrict__ nbrs, int k, int64
   }
 }
 
+// points in value-slot order: seeds keep their place, refined slot q holds point sched[q]
+template <typename T>
+__global__ void sched_points_kernel(const T* __restrict__ pts, int d, int64_t n0, int64_t n,
+                                    const int32_t* __restrict__ sched, T* __restrict__ out) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * d; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = e / d;
+    const int j = (int)(e - s * d);
+    const int64_t g = s < n0 ? s : n0 + sched[s - n0];
+    out[e] = pts[g * d + j];
+  }
+}
+
 struct SchedWs {
   uint64_t *key0, *key1;
   uint32_t *pay0, *pay1, *slot;
@@ -123,7 +137,8 @@ bool carve_sched(Workspace& w, int64_t n_ref, SchedWs& b) {
 
 template <typename T>
 int schedule_impl(const T* pts, int64_t n, int d, const int32_t* nbrs, int64_t n0, int k, const int64_t* offsets,
-                  int64_t n_off, int32_t* sched, int32_t* nbv, void* ws, size_t ws_bytes, hipStream_t s) {
+                  int64_t n_off, int32_t* sched, int32_t* nbv, T* pts_sched, void* ws, size_t ws_bytes,
+                  hipStream_t s) {
   using K = typename KeyOf<T>::type;
   const int64_t n_ref = n - n0;
   GGP_CHECK_ARG(d >= 1 && d <= 8 && k >= 1 && n0 >= 1 && n_ref >= 0, "graph_schedule: bad sizes");
@@ -158,6 +173,9 @@ int schedule_impl(const T* pts, int64_t n, int d, const int32_t* nbrs, int64_t n
   hipLaunchKernelGGL(sched_out_kernel, dim3(grid_capped(n_ref, 256)), dim3(256), 0, s, pay, n_ref, sched, b.slot);
   hipLaunchKernelGGL(sched_nbrs_kernel, dim3(grid_capped(n_ref * k, 256)), dim3(256), 0, s, nbrs, k, n0, n_ref,
                      (const int32_t*)sched, (const uint32_t*)b.slot, nbv);
+  if (pts_sched)
+    hipLaunchKernelGGL(sched_points_kernel<T>, dim3(grid_capped(n * d, 256)), dim3(256), 0, s, pts, d, n0, n,
+                       (const int32_t*)sched, pts_sched);
   GGP_LAUNCH_CHECK();
   return kOk;
 }
@@ -172,12 +190,14 @@ size_t graph_schedule_workspace(int64_t n_ref) {
 }
 
 int graph_schedule(const void* pts, int dtype, int64_t n, int d, const int32_t* nbrs, int64_t n0, int k,
-                   const int64_t* offsets, int64_t n_off, int32_t* sched, int32_t* nbv, void* ws, size_t ws_bytes,
-                   hipStream_t s) {
+                   const int64_t* offsets, int64_t n_off, int32_t* sched, int32_t* nbv, void* pts_sched, void* ws,
+                   size_t ws_bytes, hipStream_t s) {
   if (dtype == 0)
-    return schedule_impl<float>((const float*)pts, n, d, nbrs, n0, k, offsets, n_off, sched, nbv, ws, ws_bytes, s);
+    return schedule_impl<float>((const float*)pts, n, d, nbrs, n0, k, offsets, n_off, sched, nbv, (float*)pts_sched,
+                                ws, ws_bytes, s);
   if (dtype == 1)
-    return schedule_impl<double>((const double*)pts, n, d, nbrs, n0, k, offsets, n_off, sched, nbv, ws, ws_bytes, s);
+    return schedule_impl<double>((const double*)pts, n, d, nbrs, n0, k, offsets, n_off, sched, nbv,
+                                 (double*)pts_sched, ws, ws_bytes, s);
   set_error("graph_schedule: dtype must be 0 (float32) or 1 (float64)");
   return kInvalidArgument;
 }

@@ -78,8 +78,9 @@ def generate(graph: Graph, covariance, xi, *, cuda: bool = False, fast_jit: bool
 
 
 # Graphs from this many refined points on get a dataflow schedule (ggp_graph_schedule):
-# ordering every batch by space makes a tile's neighbour values share cache lines.  It
-# costs one sort and an (n - n0, k) int32 array per graph, kept on the Graph.
+# ordering every batch by space makes a tile's neighbour values and coordinates share
+# cache lines.  It costs one sort, an (n - n0, k) int32 array and a copy of the points per
+# graph, kept on the Graph.
 SCHEDULE_MIN_REFINED = 1 << 20
 
 
@@ -99,13 +100,14 @@ def _schedule(graph: Graph, pts, nb):
     off = torch.tensor(graph.offsets, dtype=torch.int64, device=pts.device)
     order = torch.empty(n_ref, dtype=torch.int32, device=pts.device)
     nbv = torch.empty((n_ref, k), dtype=torch.int32, device=pts.device)
+    pts_s = torch.empty_like(pts)
     L = _lib.lib()
     ws = _lib.workspace(L.ggp_graph_schedule_workspace(n, n0))
     _lib.check(L.ggp_graph_schedule(_lib.ptr(pts), _lib.dtype_code(pts), n, d, _lib.ptr(nb), n0, k, _lib.ptr(off),
-                                    off.numel(), _lib.ptr(order), _lib.ptr(nbv), _lib.ptr(ws), ws.numel(),
-                                    _lib.stream()), "graph schedule")
-    graph._ggp_schedule = (key, (order, nbv))
-    return order, nbv
+                                    off.numel(), _lib.ptr(order), _lib.ptr(nbv), _lib.ptr(pts_s), _lib.ptr(ws),
+                                    ws.numel(), _lib.stream()), "graph schedule")
+    graph._ggp_schedule = (key, (order, nbv, pts_s))
+    return order, nbv, pts_s
 
 
 def _generate_raw(pts, nb, perm, bins, vals, x, prec, return_counters=False, sched=None):
@@ -119,8 +121,8 @@ def _generate_raw(pts, nb, perm, bins, vals, x, prec, return_counters=False, sch
             _lib.ptr(vals), bins.numel(), _lib.ptr(x), _lib.dtype_code(x), _lib.ptr(out), _lib.dtype_code(out),
             _lib.PREC[prec])
     if sched is not None:
-        _lib.check(L.ggp_generate_sched(*args, _lib.ptr(sched[0]), _lib.ptr(sched[1]), _lib.ptr(ws), ws.numel(),
-                                        _lib.stream()), "generate")
+        _lib.check(L.ggp_generate_sched(*args, _lib.ptr(sched[0]), _lib.ptr(sched[1]), _lib.ptr(sched[2]),
+                                        _lib.ptr(ws), ws.numel(), _lib.stream()), "generate")
     else:
         _lib.check(L.ggp_generate(*args, _lib.ptr(ws), ws.numel(), _lib.stream()), "generate")
     return (out, _counters(ws)) if return_counters else out

@@ -115,21 +115,23 @@ int ggp_generate(const void* points, int pdtype, int64_t n, int d, const int32_t
  * int64 array of n_offsets = batches + 1 entries) ordered by the Morton code of their
  * points.  schedule (n - n0) holds refined indices; schedule_neighbors (n - n0, k) the
  * neighbours of schedule[q] as value slots (a seed keeps its index, refined point r moves
- * to n0 + its slot).  At most 65535 batches.  Computed once per graph. */
+ * to n0 + its slot); schedule_points (n, d, dtype pdtype; may be NULL) the points in slot
+ * order.  At most 65535 batches.  Computed once per graph. */
 size_t ggp_graph_schedule_workspace(int64_t n, int64_t n0);
 int ggp_graph_schedule(const void* points, int pdtype, int64_t n, int d, const int32_t* neighbors, int64_t n0, int k,
                        const int64_t* offsets, int64_t n_offsets, int32_t* schedule, int32_t* schedule_neighbors,
-                       void* workspace, size_t workspace_bytes, void* stream);
+                       void* schedule_points, void* workspace, size_t workspace_bytes, void* stream);
 
-/* generate with a schedule (no reference counterpart; same results as ggp_generate):
- * in GGP_PREC_FP32_INC the dataflow over stored weights takes each batch in schedule order
- * and keeps its values in that order, so a tile's neighbours share cache lines.  Other
- * precisions ignore the schedule. */
+/* generate with a schedule (no reference counterpart; same results as ggp_generate): in
+ * GGP_PREC_FP32_INC all three phases run in slot space (weights and FP64 recomputation
+ * over schedule_points / schedule_neighbors, then the dataflow in schedule order with its
+ * values kept in that order), so a tile's neighbours share cache lines.  Other precisions
+ * ignore the schedule. */
 int ggp_generate_sched(const void* points, int pdtype, int64_t n, int d, const int32_t* neighbors, int64_t n0, int k,
                        const int32_t* indices, const double* cov_bins, const double* cov_vals, int n_bins,
                        const void* xi, int xi_dtype, void* values, int values_dtype, int precision,
-                       const int32_t* schedule, const int32_t* schedule_neighbors, void* workspace,
-                       size_t workspace_bytes, void* stream);
+                       const int32_t* schedule, const int32_t* schedule_neighbors, const void* schedule_points,
+                       void* workspace, size_t workspace_bytes, void* stream);
 
 /* generate_inv: graphgp.generate_inv (refine.py:162-183), incl. graphgp_cuda.refine_inv
  * (refine.py:216) and generate_dense_inv (refine.py:186-195).  values (n) -> xi (n). */

@@ -390,7 +390,7 @@ def test_scheduled_generate_equals_unscheduled(monkeypatch):
     c = gp.generate(g, cov, xi, precision="fp32_increment")  # cached schedule
     assert torch.equal(a.view(torch.int32), b.view(torch.int32))
     assert torch.equal(a.view(torch.int32), c.view(torch.int32))
-    order, nbv = g._ggp_schedule[1]
+    order, nbv, pts_s = g._ggp_schedule[1]
     o = cpu(order).astype(np.int64)
     n0 = g.offsets[0]
     assert np.array_equal(np.sort(o), np.arange(len(o)))
@@ -401,3 +401,5 @@ def test_scheduled_generate_equals_unscheduled(monkeypatch):
     nb = cpu(g.neighbors).astype(np.int64)[o]
     expect = np.where(nb < n0, nb, n0 + slot[np.maximum(nb - n0, 0)])
     assert np.array_equal(cpu(nbv), expect)
+    P = cpu(g.points)
+    assert np.array_equal(cpu(pts_s), np.concatenate([P[:n0], P[n0 + o]]))
