@@ -609,8 +609,19 @@ int refine_flow(const RefineProblem& p, int mode, const RefineIO& io, void* ws, 
     a.xi = io.xi;
     a.xi_f64 = io.xi_f64;
     a.out_f64 = io.out_f64;
-    if (two_phase<T>(p, mode)) GGP_TRY(run_two_phase<T>(a, p, b, s));
-    else GGP_TRY(run_kernel<T>(a, p, b, s));
+    if (two_phase<T>(p, mode)) {
+      GGP_TRY(run_two_phase<T>(a, p, b, s));
+    } else if (p.sched != nullptr && !generic_config(p)) {
+      // the fused dataflow in slot space: points and neighbours in schedule order, values
+      // stored in slot order, xi and output through the schedule
+      RefineProblem pw = p;
+      pw.pts = p.sched_pts;
+      pw.nbrs = p.sched_nbrs;
+      a.sched = p.sched;
+      GGP_TRY(run_kernel<T>(a, pw, b, s));
+    } else {
+      GGP_TRY(run_kernel<T>(a, p, b, s));
+    }
     if (p.with_seed) {
       hipLaunchKernelGGL(nan_fill_kernel, dim3(grid_for(std::min<int64_t>(p.n, 1 << 20), 256)), dim3(256), 0, s,
                          io.out, io.out_f64, p.n, b.ctr + 1);

@@ -633,7 +633,9 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
     T xi = T(0);
     int64_t dst = fine;
     if (valid && owner) {
-      dst = a.perm ? (int64_t)a.perm[fine] : fine;
+      // in slot space (a.sched) the point's graph position comes from the schedule
+      const int64_t g = a.sched != nullptr ? a.n0 + (int64_t)a.sched[pt] : fine;
+      dst = a.perm ? (int64_t)a.perm[g] : g;
       xi = load_as<T>(a.xi, a.xi_f64, dst - a.io_base);
     }
     bool done = !valid;

@@ -73,7 +73,7 @@ def generate(graph: Graph, covariance, xi, *, cuda: bool = False, fast_jit: bool
         from .autograd import GenerateFunction
 
         return GenerateFunction.apply(x, vals, pts, nb, perm, bins, prec)
-    sched = _schedule(graph, pts, nb) if prec == "fp32_increment" else None
+    sched = _schedule(graph, pts, nb)
     return _generate_raw(pts, nb, perm, bins, vals, x, prec, return_counters, sched)
 
 
@@ -97,12 +97,17 @@ def _schedule(graph: Graph, pts, nb):
     if cached is not None and cached[0] == key:
         return cached[1]
     n0 = n - n_ref
+    L = _lib.lib()
+    kept = n_ref * (k + 1) * 4 + pts.numel() * pts.element_size()
+    ws_bytes = L.ggp_graph_schedule_workspace(n, n0)
+    free, _ = torch.cuda.mem_get_info(pts.device)
+    if kept + ws_bytes > free // 2:  # a large graph near the memory limit runs in graph order
+        return None
     off = torch.tensor(graph.offsets, dtype=torch.int64, device=pts.device)
     order = torch.empty(n_ref, dtype=torch.int32, device=pts.device)
     nbv = torch.empty((n_ref, k), dtype=torch.int32, device=pts.device)
     pts_s = torch.empty_like(pts)
-    L = _lib.lib()
-    ws = _lib.workspace(L.ggp_graph_schedule_workspace(n, n0))
+    ws = _lib.workspace(ws_bytes)
     _lib.check(L.ggp_graph_schedule(_lib.ptr(pts), _lib.dtype_code(pts), n, d, _lib.ptr(nb), n0, k, _lib.ptr(off),
                                     off.numel(), _lib.ptr(order), _lib.ptr(nbv), _lib.ptr(pts_s), _lib.ptr(ws),
                                     ws.numel(), _lib.stream()), "graph schedule")

@@ -122,11 +122,11 @@ int ggp_graph_schedule(const void* points, int pdtype, int64_t n, int d, const i
                        const int64_t* offsets, int64_t n_offsets, int32_t* schedule, int32_t* schedule_neighbors,
                        void* schedule_points, void* workspace, size_t workspace_bytes, void* stream);
 
-/* generate with a schedule (no reference counterpart; same results as ggp_generate): in
- * GGP_PREC_FP32_INC all three phases run in slot space (weights and FP64 recomputation
- * over schedule_points / schedule_neighbors, then the dataflow in schedule order with its
- * values kept in that order), so a tile's neighbours share cache lines.  Other precisions
- * ignore the schedule. */
+/* generate with a schedule (no reference counterpart; same results as ggp_generate): the
+ * refinement runs in slot space (points and neighbours from schedule_points /
+ * schedule_neighbors, values kept in schedule order, xi and output through the schedule),
+ * so a tile's neighbours share cache lines.  In GGP_PREC_FP32_INC all three phases do; the
+ * configurations beyond k + 1 = 40 (or d > 8) ignore the schedule. */
 int ggp_generate_sched(const void* points, int pdtype, int64_t n, int d, const int32_t* neighbors, int64_t n0, int k,
                        const int32_t* indices, const double* cov_bins, const double* cov_vals, int n_bins,
                        const void* xi, int xi_dtype, void* values, int values_dtype, int precision,

@@ -403,3 +403,10 @@ def test_scheduled_generate_equals_unscheduled(monkeypatch):
     assert np.array_equal(cpu(nbv), expect)
     P = cpu(g.points)
     assert np.array_equal(cpu(pts_s), np.concatenate([P[:n0], P[n0 + o]]))
+    # the fused kernels in slot space (plain FP32 and FP64 generate)
+    monkeypatch.setattr(R, "SCHEDULE_MIN_REFINED", 1 << 40)
+    ref = {pr: gp.generate(g, cov, xi, precision=pr) for pr in ("fp32", "fp64")}
+    monkeypatch.setattr(R, "SCHEDULE_MIN_REFINED", 0)
+    for pr in ("fp32", "fp64"):
+        v = gp.generate(g, cov, xi, precision=pr)
+        assert torch.equal(v, ref[pr]), pr
