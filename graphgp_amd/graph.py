@@ -63,8 +63,52 @@ def _offsets(depths_sorted: torch.Tensor, max_depth: int) -> Tuple[int, ...]:
     return tuple(int(x) for x in _lib.to_host(off).tolist())
 
 
-def build_graph(points, *, n0: int, k: int, cuda: bool = False, return_depths: bool = False) -> Graph:
-    """Graph of each point's k nearest preceding neighbours in k-d tree order (graph.py:73-98)."""
+# Graphs from this many refined points on get a dataflow schedule (ggp_graph_schedule):
+# ordering every batch by space makes a tile's neighbour values and coordinates share
+# cache lines.  It costs one sort, an (n - n0, k) int32 array and a copy of the points per
+# graph, kept on the Graph.
+SCHEDULE_MIN_REFINED = 1 << 20
+
+
+def _schedule(graph: Graph, pts, nb):
+    """The graph's dataflow schedule, computed on first use and kept on the graph (its
+    arrays are treated as immutable, like the reference's JAX arrays); None for small
+    graphs or more than 65535 batches."""
+    n, d = pts.shape
+    n_ref, k = nb.shape
+    if n_ref < SCHEDULE_MIN_REFINED or len(graph.offsets) > 65536:
+        return None
+    key = (pts.data_ptr(), nb.data_ptr(), tuple(graph.offsets))
+    cached = getattr(graph, "_ggp_schedule", None)
+    if cached is not None and cached[0] in (key, "off"):
+        return cached[1]
+    n0 = n - n_ref
+    L = _lib.lib()
+    kept = n_ref * (k + 1) * 4 + pts.numel() * pts.element_size()
+    ws_bytes = L.ggp_graph_schedule_workspace(n, n0)
+    free, _ = torch.cuda.mem_get_info(pts.device)
+    if kept + ws_bytes > free // 2:  # a large graph near the memory limit runs in graph order
+        return None
+    # pinned and non-blocking: a serving thread must not wait here for its stream's work
+    off = torch.tensor(graph.offsets, dtype=torch.int64).pin_memory().to(pts.device, non_blocking=True)
+    order = torch.empty(n_ref, dtype=torch.int32, device=pts.device)
+    nbv = torch.empty((n_ref, k), dtype=torch.int32, device=pts.device)
+    pts_s = torch.empty_like(pts)
+    ws = _lib.workspace(ws_bytes)
+    _lib.check(L.ggp_graph_schedule(_lib.ptr(pts), _lib.dtype_code(pts), n, d, _lib.ptr(nb), n0, k, _lib.ptr(off),
+                                    off.numel(), _lib.ptr(order), _lib.ptr(nbv), _lib.ptr(pts_s), _lib.ptr(ws),
+                                    ws.numel(), _lib.stream()), "graph schedule")
+    graph._ggp_schedule = (key, (order, nbv, pts_s))
+    return order, nbv, pts_s
+
+
+def build_graph(points, *, n0: int, k: int, cuda: bool = False, return_depths: bool = False,
+                schedule: bool = True) -> Graph:
+    """Graph of each point's k nearest preceding neighbours in k-d tree order (graph.py:73-98).
+
+    ``schedule`` (not in the reference): for large graphs, also compute the dataflow
+    schedule that ``generate`` uses (an order of evaluation; results are identical without
+    it).  ``False`` skips it for this graph, saving its memory."""
     p = as_points(points)
     n, d = p.shape
     if n0 < k:
@@ -85,6 +129,10 @@ def build_graph(points, *, n0: int, k: int, cuda: bool = False, return_depths: b
                "build_graph")
     del ws
     g = Graph(pts, nbrs, _offsets(depths, mx.value), idx)
+    if schedule:
+        _schedule(g, pts, nbrs)  # large graphs: the dataflow schedule, while the build has the stream
+    else:
+        g._ggp_schedule = ("off", None)
     return (g, depths) if return_depths else g
 
 

@@ -23,7 +23,7 @@ import torch
 
 from . import _lib
 from ._arrays import as_covariance, as_device, as_index, as_points
-from .graph import Graph
+from .graph import Graph, _schedule
 
 PRECISIONS = tuple(_lib.PREC)
 
@@ -75,44 +75,6 @@ def generate(graph: Graph, covariance, xi, *, cuda: bool = False, fast_jit: bool
         return GenerateFunction.apply(x, vals, pts, nb, perm, bins, prec)
     sched = _schedule(graph, pts, nb)
     return _generate_raw(pts, nb, perm, bins, vals, x, prec, return_counters, sched)
-
-
-# Graphs from this many refined points on get a dataflow schedule (ggp_graph_schedule):
-# ordering every batch by space makes a tile's neighbour values and coordinates share
-# cache lines.  It costs one sort, an (n - n0, k) int32 array and a copy of the points per
-# graph, kept on the Graph.
-SCHEDULE_MIN_REFINED = 1 << 20
-
-
-def _schedule(graph: Graph, pts, nb):
-    """The graph's dataflow schedule, computed on first use and kept on the graph (its
-    arrays are treated as immutable, like the reference's JAX arrays); None for small
-    graphs or more than 65535 batches."""
-    n, d = pts.shape
-    n_ref, k = nb.shape
-    if n_ref < SCHEDULE_MIN_REFINED or len(graph.offsets) > 65536:
-        return None
-    key = (pts.data_ptr(), nb.data_ptr(), tuple(graph.offsets))
-    cached = getattr(graph, "_ggp_schedule", None)
-    if cached is not None and cached[0] == key:
-        return cached[1]
-    n0 = n - n_ref
-    L = _lib.lib()
-    kept = n_ref * (k + 1) * 4 + pts.numel() * pts.element_size()
-    ws_bytes = L.ggp_graph_schedule_workspace(n, n0)
-    free, _ = torch.cuda.mem_get_info(pts.device)
-    if kept + ws_bytes > free // 2:  # a large graph near the memory limit runs in graph order
-        return None
-    off = torch.tensor(graph.offsets, dtype=torch.int64, device=pts.device)
-    order = torch.empty(n_ref, dtype=torch.int32, device=pts.device)
-    nbv = torch.empty((n_ref, k), dtype=torch.int32, device=pts.device)
-    pts_s = torch.empty_like(pts)
-    ws = _lib.workspace(ws_bytes)
-    _lib.check(L.ggp_graph_schedule(_lib.ptr(pts), _lib.dtype_code(pts), n, d, _lib.ptr(nb), n0, k, _lib.ptr(off),
-                                    off.numel(), _lib.ptr(order), _lib.ptr(nbv), _lib.ptr(pts_s), _lib.ptr(ws),
-                                    ws.numel(), _lib.stream()), "graph schedule")
-    graph._ggp_schedule = (key, (order, nbv, pts_s))
-    return order, nbv, pts_s
 
 
 def _generate_raw(pts, nb, perm, bins, vals, x, prec, return_counters=False, sched=None):

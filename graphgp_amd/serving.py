@@ -35,7 +35,8 @@ GiB = 1 << 30
 
 # ---------------------------------------------------------------- byte accounting
 def graph_bytes(n: int, d: int, k: int, n0: int, point_bytes: int = 4) -> int:
-    """Device bytes of a resident Graph: points, neighbours, indices."""
+    """Device bytes of a resident Graph: points, neighbours, indices (the server builds
+    its graphs without a dataflow schedule)."""
     return n * d * point_bytes + (n - n0) * k * 4 + n * 4
 
 
@@ -217,7 +218,8 @@ class Server:
         self.adm = Admission(hbm_budget_bytes, self.cache)
         self.small_n = small_n
         self.max_inflight = max_inflight
-        self.build_fn = build_fn or (lambda pts, n0, k: gp.build_graph(pts, n0=n0, k=k))
+        # no dataflow schedules: many resident graphs, and builds dominate the mix
+        self.build_fn = build_fn or (lambda pts, n0, k: gp.build_graph(pts, n0=n0, k=k, schedule=False))
         small_work, small_build, large = torch.cuda.Stream(), torch.cuda.Stream(), torch.cuda.Stream()
         self.work = [small_work, large]  # [small, large]
         self.build_streams = [small_build, large]
@@ -504,8 +506,17 @@ def serve_benchmark(n_requests: int = 2000, seed: int = 5, load: float = 0.85,
         srv.close()
         return out
 
+    def fresh():  # every run starts from an empty device cache
+        import gc
+
+        gc.collect()
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+
     full = batch_run(1.0)
+    fresh()
     batch = batch_run(large_cu_fraction)
+    fresh()
     rate = load * batch["requests"] / batch["wall_s"]
     reqs = materialize(synthetic_mix(n_requests, rate_per_s=rate, seed=seed))
     torch.cuda.synchronize()

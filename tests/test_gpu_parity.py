@@ -378,7 +378,7 @@ def test_scheduled_generate_equals_unscheduled(monkeypatch):
     # values; the schedule is a permutation that keeps the batch order
     import sys
 
-    R = sys.modules["graphgp_amd.refine"]  # the module (the package exports a function of that name)
+    R = sys.modules["graphgp_amd.graph"]
     p = clustered_points(200_000, d=3, seed=31)
     g = gp.build_graph(p, n0=1000, k=30)
     cov = gp.extras.matern_kernel(p=1, variance=1.0, cutoff=0.05, r_min=1e-6, r_max=2, n_bins=2001, jitter=1e-5)
