@@ -327,9 +327,12 @@ __device__ __forceinline__ double fix64(const RefineArgs<T>& a, int64_t pt, bool
 }
 
 // ------------------------------------------------------------------ the kernel
-template <typename T, int KK, int P, int D, bool FAST>
+// MODE >= 0 fixes the mode at compile time (the code of the other modes is dropped);
+// MODE = -1 takes it from a.mode.
+template <typename T, int KK, int P, int D, bool FAST, int MODE = -1>
 __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
   static_assert(KK % P == 0, "KK must be a multiple of P");
+  const int mode = MODE >= 0 ? MODE : a.mode;
   constexpr int R = KK / P;
   constexpr int S = P * R * (R + 1) / 2;
   constexpr int GPT = 256 / P;
@@ -363,7 +366,7 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
 
   const int lane = threadIdx.x & (P - 1);
   const int grp = threadIdx.x / P;
-  const int k = a.k;
+  const int k_arg = a.k;
   const bool incr = a.increment != 0;
   const bool fixing = a.tab64 != nullptr;
   const T tau = (T)a.fix_tau;
@@ -379,6 +382,11 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
     const int64_t tile = s_tile;
     __syncthreads();
     if (tile >= a.ntiles) break;
+    // k anew in every tile, opaque to the compiler (an SGPR it may not look through): the
+    // unrolled column tests below are then scalar compares where they are used, instead of
+    // being hoisted out of the tile loop as one lane mask each, which spill.
+    int k = k_arg;
+    asm volatile("" : "+s"(k));
     int64_t pt = tile * GPT + grp;
     bool valid = pt < a.n_ref;
     const int64_t slot_in = pt;
@@ -485,7 +493,7 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
     // FP64 recomputation of the tile's flagged points before any waiting: the flagged
     // groups are listed in LDS and the workgroup's four wavefronts take them in turn
     double sig64 = 1.0;
-    if (fixing && a.mode != kWeights) {
+    if (fixing && mode != kWeights) {
       if (threadIdx.x == 0) s_nfix = 0;
       __syncthreads();
       if (flagged && lane == 0) s_fixlist[atomicAdd(&s_nfix, 1)] = grp;
@@ -494,7 +502,7 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
       for (int it = threadIdx.x >> 6; it < nfix; it += 4) {
         const int g = s_fixlist[it];
         double* wslot = reinterpret_cast<double*>(smem_raw + a.fix_lds_offset) + 4 * (KK * (KK + 1)) + g * KK;
-        const double sg = fix64<T, D, FAST>(a, tile * GPT + g, a.mode != kLogdet, slab, KK + 1, wslot);
+        const double sg = fix64<T, D, FAST>(a, tile * GPT + g, mode != kLogdet, slab, KK + 1, wslot);
         if ((threadIdx.x & 63) == 0) {
           s_fixsig[g] = sg;
           atomicAdd(a.fix_count, 1u);
@@ -503,7 +511,7 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
       __syncthreads();
       if (flagged) sig64 = s_fixsig[grp];
     }
-    if (a.mode == kWeights) {
+    if (mode == kWeights) {
       // w = L_A^{-T} z with z = L[k, :k] (so that mean = z . L_A^{-1} u = w . u): with
       // w_k = -1, w_i = -(sum_{i<m<=k} L[m][i] w_m) / L[i][i], rows m spread over the
       // group's lanes, one group sum per column
@@ -585,7 +593,7 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
       return uk;
     };
 
-    if (a.mode == kLogdet) {
+    if (mode == kLogdet) {
       double ld = 0.0;
       if (valid && owner) ld = flagged ? log(sig64) : 0.5 * log((double)skk);
       if (valid && owner && ld != ld) atomicOr(&a.flags[0], 1u);
@@ -598,7 +606,7 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
       continue;
     }
 
-    if (a.mode == kInverse) {
+    if (mode == kInverse) {
       T u[R];
       T v0 = T(0);
 #pragma unroll

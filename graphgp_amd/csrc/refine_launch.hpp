@@ -6,9 +6,9 @@ namespace ggp {
 
 size_t refine_lds_limit();
 
-template <typename T, int KK, int P, int D, bool FAST>
+template <typename T, int KK, int P, int D, bool FAST, int MODE>
 int launch_variant(const RefineArgs<T>& a, hipStream_t s, size_t lds_bytes, int grid) {
-  auto kern = refine_kernel<T, KK, P, D, FAST>;
+  auto kern = refine_kernel<T, KK, P, D, FAST, MODE>;
   if (lds_bytes > 65536) {
     hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     if (e != hipSuccess) {
@@ -42,8 +42,12 @@ int launch_config(RefineArgs<T> a, hipStream_t s, size_t lds_bytes, int grid) {
     set_error("refine: covariance table does not fit in LDS (" + std::to_string(lds_bytes) + " bytes)");
     return kUnsupported;
   }
-  GGP_TRY((launch_variant<T, KK, P, D, true>(a, s, lds_bytes, grid)));
-  return launch_variant<T, KK, P, D, false>(a, s, lds_bytes, grid);
+  if (a.mode == kWeights) {  // the weights phases: an instantiation without the other modes' code
+    GGP_TRY((launch_variant<T, KK, P, D, true, kWeights>(a, s, lds_bytes, grid)));
+    return launch_variant<T, KK, P, D, false, kWeights>(a, s, lds_bytes, grid);
+  }
+  GGP_TRY((launch_variant<T, KK, P, D, true, -1>(a, s, lds_bytes, grid)));
+  return launch_variant<T, KK, P, D, false, -1>(a, s, lds_bytes, grid);
 }
 
 
