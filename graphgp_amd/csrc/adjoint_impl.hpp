@@ -76,7 +76,7 @@ __global__ __launch_bounds__(256, 2) void adjoint_kernel(AdjointArgs<T> a) {
   const float lb1 = meta.lb1, inv_step = meta.inv_step;
   const int lane = threadIdx.x & (P - 1);
   const int grp = threadIdx.x / P;
-  const int k = a.k;
+  const int k_arg = a.k;
 
   // segment of r (same rule as the forward lookup) and its value
   auto seg_of = [&](T r, T r2) -> int {
@@ -126,6 +126,10 @@ __global__ __launch_bounds__(256, 2) void adjoint_kernel(AdjointArgs<T> a) {
     const int64_t tile = s_tile;
     __syncthreads();
     if (tile >= a.ntiles) break;
+    // k anew in every tile, opaque to the compiler: the unrolled column tests stay scalar
+    // compares where they are used instead of hoisted lane masks, which spill
+    int k = k_arg;
+    asm volatile("" : "+s"(k));
     // tiles in decreasing index order
     const int64_t pt = a.n_ref - 1 - (tile * GPT + grp);
     const bool valid = pt >= 0;
