@@ -103,16 +103,18 @@ __device__ __forceinline__ void static_for(F&& f) {
   }
 }
 
-// Broadcast lane S of every P-lane group to the whole group.
+// Broadcast lane S of every P-lane group to the whole group.  The quad_perm sources are
+// always inside the lane's own quad, so no lane reads an invalid one; with bound_ctrl set
+// the DPP move needs no old value, and no copy into its destination first.
 template <int P, int S>
 __device__ __forceinline__ int bcast_i32(int x) {
   if constexpr (P == 1) {
     return x;
   } else if constexpr (P == 2) {
     constexpr int q = S | (S << 2) | ((2 + S) << 4) | ((2 + S) << 6);
-    return __builtin_amdgcn_update_dpp(0, x, q, 0xF, 0xF, false);
+    return __builtin_amdgcn_update_dpp(0, x, q, 0xF, 0xF, true);
   } else if constexpr (P == 4) {
-    return __builtin_amdgcn_update_dpp(0, x, S * 0x55, 0xF, 0xF, false);
+    return __builtin_amdgcn_update_dpp(0, x, S * 0x55, 0xF, 0xF, true);
   } else if constexpr (P == 8) {
     return __builtin_amdgcn_ds_swizzle(x, 0x18 | (S << 5));
   } else {
