@@ -184,9 +184,12 @@ __device__ __forceinline__ T table_lookup_fast(const TabE<T>* __restrict__ tab, 
                                                int nseg, T x1, T x_last, T f_last, float lb1, float inv_step, T r,
                                                T r2) {
   if constexpr (sizeof(T) == 4) {
-    int j = log_guess((float)r2, lb1, inv_step);
-    j = j < 1 ? 1 : (j > nseg - 1 ? nseg - 1 : j);
-    j = (r < x1) ? 0 : j;
+    // rounding down: below the first positive bin (r < x1, and r = 0) the guess is <= 0,
+    // so one clamp to [0, nseg - 1] also finds segment 0 (for j >= 1 this equals the
+    // certified log_guess, whose argument is then positive)
+    const float lr = 0.5f * __builtin_amdgcn_logf((float)r2);
+    int j = 1 + (int)floorf((lr - lb1) * inv_step);
+    j = max(0, min(j, nseg - 1));
     const T v = seg_value<T>(tab[j], edges[j], r);
     return (r >= x_last) ? f_last : v;
   } else {
