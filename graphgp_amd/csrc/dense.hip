@@ -68,37 +68,91 @@ __global__ void cross_cov_kernel(const void* __restrict__ pa, int64_t na, const 
   }
 }
 
-// Factorise the diagonal block D (nbj x nbj, lower, in LDS) with 256 threads and write
-// its inverse to Di (lower).  Four threads share a row of the trailing update.
-__device__ inline void factor_invert_lds(double (*D)[NB + 1], double (*Di)[NB + 1], int nbj) {
+// T = the nr x nc block of A at (r0, c0), zero-padded to 64 x 64.  Every thread issues its
+// sixteen loads (clamped addresses) before the first LDS store, so the tile costs one
+// memory round trip.
+__device__ __forceinline__ void load_tile(double (*T)[NB + 1], const double* __restrict__ A, int64_t n, int64_t r0,
+                                          int64_t c0, int nr, int nc) {
+  double v[NB * NB / 256];
+#pragma unroll
+  for (int u = 0; u < NB * NB / 256; ++u) {
+    const int q = threadIdx.x + 256 * u, r = q / NB, c = q % NB;
+    v[u] = A[(r0 + min(r, nr - 1)) * n + c0 + min(c, nc - 1)];
+  }
+#pragma unroll
+  for (int u = 0; u < NB * NB / 256; ++u) {
+    const int q = threadIdx.x + 256 * u, r = q / NB, c = q % NB;
+    T[r][c] = (r < nr && c < nc) ? v[u] : 0.0;
+  }
+}
+
+// Factorise the diagonal block D (nbj x nbj, lower, in LDS) and write its inverse to Di.
+// Rows beyond nbj are padded with the identity.  Lane r of wavefront w holds row r's
+// columns 16w .. 16w + 15 of the trailing matrix (a) and of the inverse being formed (b)
+// in registers.  Column c is published by its owning wavefront, together with row c of
+// b, in a double-buffered LDS exchange (one workgroup barrier per column): every lane
+// then applies the rank-1 update a[r][m] -= a[r][c] a[m][c] / a[c][c] and the matching
+// forward-substitution step b[r] -= a[r][c] b[c] / a[c][c], so L^{-1} comes out of the
+// same sweep (row r of L^{-1} is b[r] / L[r][r]).  A column's unscaled values go to D as
+// it is published and its registers are not read again, so the update runs over all
+// sixteen registers unmasked; the square roots and the scaling follow in one pass.
+__device__ __forceinline__ void factor_invert_lds(double (*D)[NB + 1], double (*Di)[NB + 1], double* xbuf, int nbj) {
+  static_assert(NB == 64, "one lane per row of the block");
+  constexpr int Q = NB / 4;
   const int t = threadIdx.x;
-  const int row = t & (NB - 1), lane4 = t >> 6;
-  for (int c = 0; c < nbj; ++c) {
-    __syncthreads();
-    const double dc = sqrt(D[c][c]);
-    __syncthreads();
-    if (t < nbj && t > c) D[t][c] /= dc;
-    if (t == c) D[c][c] = dc;
-    __syncthreads();
-    if (row > c && row < nbj) {
-      const double l = D[row][c];
-      for (int m = c + 1 + lane4; m <= row; m += 4) D[row][m] -= l * D[m][c];
+  const int row = t & (NB - 1);
+  const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+  __syncthreads();
+  double a[Q], b[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int m = Q * w + q;
+    a[q] = row < nbj ? (m < nbj ? D[row][m] : 0.0) : (m == row ? 1.0 : 0.0);
+    b[q] = m == row ? 1.0 : 0.0;
+  }
+  __syncthreads();  // D is rewritten column by column below
+  // xbuf: 2 x [column c (64) | row c of b (64)]
+  for (int c0 = 0; c0 < NB; c0 += Q) {
+    const bool owner = w == c0 / Q;
+#pragma unroll
+    for (int qc = 0; qc < Q; ++qc) {
+      const int c = c0 + qc;
+      double* cb = xbuf + (qc & 1) * 2 * NB;
+      double* rb = cb + NB;
+      if (owner) cb[row] = a[qc];
+      if (row == c) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) rb[Q * w + q] = b[q];
+      }
+      __syncthreads();
+      double cm[Q], rq[Q];
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        cm[q] = cb[Q * w + q];
+        rq[q] = rb[Q * w + q];
+      }
+      const double dcc = cb[c];
+      const double x = cb[row];
+      const double s = row > c ? x / dcc : 0.0;
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        a[q] -= s * cm[q];
+        b[q] -= s * rq[q];
+      }
+      if (owner) D[row][c] = row >= c ? x : 0.0;
     }
   }
   __syncthreads();
-  // inverse: thread j solves L x = e_j (x_i = 0 for i < j)
-  if (t < NB) {
-    const int j = t;
-    double x[NB];
+  double* dg = xbuf;
+  if (t < NB) dg[t] = sqrt(D[t][t]);
+  __syncthreads();
+  const double dr = dg[row];
 #pragma unroll
-    for (int i = 0; i < NB; ++i) {
-      double s = (i == j) ? 1.0 : 0.0;
-#pragma unroll
-      for (int m = 0; m < i; ++m) s -= D[i][m] * x[m];
-      x[i] = (i < nbj) ? s / D[i][i] : 0.0;
-    }
-#pragma unroll
-    for (int i = 0; i < NB; ++i) Di[i][j] = x[i];
+  for (int q = 0; q < Q; ++q) {
+    const int m = Q * w + q;
+    const double v = D[row][m];
+    D[row][m] = row > m ? v / dg[m] : (row == m ? dr : 0.0);
+    Di[row][m] = b[q] / dr;
   }
   __syncthreads();
 }
@@ -108,13 +162,11 @@ __global__ __launch_bounds__(256) void diag_factor_kernel(double* __restrict__ A
                                                           double* __restrict__ Linv) {
   __shared__ double D[NB][NB + 1];
   __shared__ double Di[NB][NB + 1];
+  __shared__ double xbuf[4 * NB];
   const int64_t j0 = (int64_t)jb * NB;
   const int nbj = (int)min<int64_t>(NB, n - j0);
-  for (int q = threadIdx.x; q < NB * NB; q += 256) {
-    const int r = q / NB, c = q % NB;
-    D[r][c] = (r < nbj && c < nbj) ? A[(j0 + r) * n + j0 + c] : 0.0;
-  }
-  factor_invert_lds(D, Di, nbj);
+  load_tile(D, A, n, j0, j0, nbj, nbj);
+  factor_invert_lds(D, Di, xbuf, nbj);
   for (int q = threadIdx.x; q < NB * NB; q += 256) {
     const int r = q / NB, c = q % NB;
     if (r < nbj && c < nbj) A[(j0 + r) * n + j0 + c] = c <= r ? D[r][c] : 0.0;
@@ -132,11 +184,8 @@ __global__ __launch_bounds__(256) void panel_gemm_kernel(double* __restrict__ A,
   const int64_t j0 = (int64_t)jb * NB, i0 = (int64_t)ib * NB;
   const int nbj = (int)min<int64_t>(NB, n - j0);
   const int t = threadIdx.x;
-  for (int q = t; q < NB * NB; q += 256) {
-    const int r = q / NB, c = q % NB;
-    X[r][c] = (i0 + r < n && c < nbj) ? A[(i0 + r) * n + j0 + c] : 0.0;
-    Li[r][c] = Linv[q];
-  }
+  load_tile(X, A, n, i0, j0, (int)min<int64_t>(NB, n - i0), nbj);
+  load_tile(Li, Linv, NB, 0, 0, NB, NB);
   __syncthreads();
   const int tr = (t / 16) * 4, tc = (t % 16) * 4;
   double acc[4][4] = {};
@@ -165,6 +214,7 @@ __global__ __launch_bounds__(256) void update_kernel(double* __restrict__ A, int
                                                      double* __restrict__ Linv) {
   __shared__ double Li[NB][NB + 1];
   __shared__ double Lm[NB][NB + 1];
+  __shared__ double xbuf[4 * NB];
   int e = blockIdx.x;
   int aa = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
   while ((aa + 1) * (aa + 2) / 2 <= e) ++aa;
@@ -174,13 +224,16 @@ __global__ __launch_bounds__(256) void update_kernel(double* __restrict__ A, int
   const int64_t j0 = (int64_t)jb * NB, i0 = (int64_t)ib * NB, m0 = (int64_t)mb * NB;
   const int nbj = (int)min<int64_t>(NB, n - j0);
   const int t = threadIdx.x;
-  for (int q = t; q < NB * NB; q += 256) {
-    const int r = q / NB, c = q % NB;
-    Li[r][c] = (i0 + r < n && c < nbj) ? A[(i0 + r) * n + j0 + c] : 0.0;
-    Lm[r][c] = (m0 + r < n && c < nbj) ? A[(m0 + r) * n + j0 + c] : 0.0;
-  }
-  __syncthreads();
   const int tr = (t / 16) * 4, tc = (t % 16) * 4;
+  // the target block's current values, fetched with the tiles (one memory round trip)
+  double old[4][4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) old[u][v] = A[min(i0 + tr + u, n - 1) * n + min(m0 + tc + v, n - 1)];
+  load_tile(Li, A, n, i0, j0, (int)min<int64_t>(NB, n - i0), nbj);
+  load_tile(Lm, A, n, m0, j0, (int)min<int64_t>(NB, n - m0), nbj);
+  __syncthreads();
   double acc[4][4] = {};
   for (int c = 0; c < nbj; ++c) {
     double li[4], lm[4];
@@ -197,7 +250,7 @@ __global__ __launch_bounds__(256) void update_kernel(double* __restrict__ A, int
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int64_t r = i0 + tr + u, c = m0 + tc + v;
-        if (r < n && c < n && c <= r) A[r * n + c] -= acc[u][v];
+        if (r < n && c < n && c <= r) A[r * n + c] = old[u][v] - acc[u][v];
       }
     return;
   }
@@ -209,9 +262,9 @@ __global__ __launch_bounds__(256) void update_kernel(double* __restrict__ A, int
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int r = tr + u, c = tc + v;
-      Li[r][c] = (r < nb2 && c < nb2 && c <= r) ? A[(i0 + r) * n + i0 + c] - acc[u][v] : 0.0;
+      Li[r][c] = (r < nb2 && c < nb2 && c <= r) ? old[u][v] - acc[u][v] : 0.0;
     }
-  factor_invert_lds(Li, Lm, nb2);
+  factor_invert_lds(Li, Lm, xbuf, nb2);
   for (int q = t; q < NB * NB; q += 256) {
     const int r = q / NB, c = q % NB;
     if (r < nb2 && c < nb2) A[(i0 + r) * n + i0 + c] = c <= r ? Li[r][c] : 0.0;
