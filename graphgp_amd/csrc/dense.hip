@@ -95,8 +95,10 @@ __device__ __forceinline__ void load_tile(double (*T)[NB + 1], const double* __r
 // forward-substitution step b[r] -= a[r][c] b[c] / a[c][c], so L^{-1} comes out of the
 // same sweep (row r of L^{-1} is b[r] / L[r][r]).  A column's unscaled values go to D as
 // it is published and its registers are not read again, so the update runs over all
-// sixteen registers unmasked; the square roots and the scaling follow in one pass.
-__device__ __forceinline__ void factor_invert_lds(double (*D)[NB + 1], double (*Di)[NB + 1], double* xbuf, int nbj) {
+// sixteen registers unmasked; the square roots and the scaling follow in one pass.  The
+// exchange buffers live in Di, which is written only after the sweep, and the square
+// roots in D's padding column, so the routine needs no LDS beyond the two tiles.
+__device__ __forceinline__ void factor_invert_lds(double (*D)[NB + 1], double (*Di)[NB + 1], int nbj) {
   static_assert(NB == 64, "one lane per row of the block");
   constexpr int Q = NB / 4;
   const int t = threadIdx.x;
@@ -111,7 +113,8 @@ __device__ __forceinline__ void factor_invert_lds(double (*D)[NB + 1], double (*
     b[q] = m == row ? 1.0 : 0.0;
   }
   __syncthreads();  // D is rewritten column by column below
-  // xbuf: 2 x [column c (64) | row c of b (64)]
+  // exchange: 2 x [column c (64) | row c of b (64)] at the start of Di
+  double* xbuf = &Di[0][0];
   for (int c0 = 0; c0 < NB; c0 += Q) {
     const bool owner = w == c0 / Q;
 #pragma unroll
@@ -134,24 +137,29 @@ __device__ __forceinline__ void factor_invert_lds(double (*D)[NB + 1], double (*
       const double dcc = cb[c];
       const double x = cb[row];
       const double s = row > c ? x / dcc : 0.0;
+      // a changes only right of the pivot (columns > c), b only at columns <= c (row c of
+      // b is zero beyond c): a wavefront whose sixteen columns lie on one side skips the
+      // other update (a uniform test)
+      if (c0 <= Q * w) {
 #pragma unroll
-      for (int q = 0; q < Q; ++q) {
-        a[q] -= s * cm[q];
-        b[q] -= s * rq[q];
+        for (int q = 0; q < Q; ++q) a[q] -= s * cm[q];
+      }
+      if (c0 >= Q * w) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) b[q] -= s * rq[q];
       }
       if (owner) D[row][c] = row >= c ? x : 0.0;
     }
   }
   __syncthreads();
-  double* dg = xbuf;
-  if (t < NB) dg[t] = sqrt(D[t][t]);
+  if (t < NB) D[t][NB] = sqrt(D[t][t]);  // padding column: never part of the block
   __syncthreads();
-  const double dr = dg[row];
+  const double dr = D[row][NB];
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     const int m = Q * w + q;
     const double v = D[row][m];
-    D[row][m] = row > m ? v / dg[m] : (row == m ? dr : 0.0);
+    D[row][m] = row > m ? v / D[m][NB] : (row == m ? dr : 0.0);
     Di[row][m] = b[q] / dr;
   }
   __syncthreads();
@@ -162,11 +170,10 @@ __global__ __launch_bounds__(256) void diag_factor_kernel(double* __restrict__ A
                                                           double* __restrict__ Linv) {
   __shared__ double D[NB][NB + 1];
   __shared__ double Di[NB][NB + 1];
-  __shared__ double xbuf[4 * NB];
   const int64_t j0 = (int64_t)jb * NB;
   const int nbj = (int)min<int64_t>(NB, n - j0);
   load_tile(D, A, n, j0, j0, nbj, nbj);
-  factor_invert_lds(D, Di, xbuf, nbj);
+  factor_invert_lds(D, Di, nbj);
   for (int q = threadIdx.x; q < NB * NB; q += 256) {
     const int r = q / NB, c = q % NB;
     if (r < nbj && c < nbj) A[(j0 + r) * n + j0 + c] = c <= r ? D[r][c] : 0.0;
@@ -214,7 +221,6 @@ __global__ __launch_bounds__(256) void update_kernel(double* __restrict__ A, int
                                                      double* __restrict__ Linv) {
   __shared__ double Li[NB][NB + 1];
   __shared__ double Lm[NB][NB + 1];
-  __shared__ double xbuf[4 * NB];
   int e = blockIdx.x;
   int aa = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
   while ((aa + 1) * (aa + 2) / 2 <= e) ++aa;
@@ -264,7 +270,7 @@ __global__ __launch_bounds__(256) void update_kernel(double* __restrict__ A, int
       const int r = tr + u, c = tc + v;
       Li[r][c] = (r < nb2 && c < nb2 && c <= r) ? old[u][v] - acc[u][v] : 0.0;
     }
-  factor_invert_lds(Li, Lm, xbuf, nb2);
+  factor_invert_lds(Li, Lm, nb2);
   for (int q = t; q < NB * NB; q += 256) {
     const int r = q / NB, c = q % NB;
     if (r < nb2 && c < nb2) A[(i0 + r) * n + i0 + c] = c <= r ? Li[r][c] : 0.0;
