@@ -489,7 +489,7 @@ def materialize(specs, device="cuda"):
     return reqs
 
 
-def serve_benchmark(n_requests: int = 2000, seed: int = 5, load: float = 0.85,
+def serve_benchmark(n_requests: int = 2000, seed: int = 5, load: float = 0.9,
                     large_cu_fraction: float = 0.35) -> Dict[str, Any]:
     """The serving measurement: the synthetic mix served all at once (batch mode, the
     server's capacity), then with Poisson arrivals at ``load`` x that request rate, both
@@ -548,6 +548,6 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser(description="serving benchmark over the synthetic request mix")
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--seed", type=int, default=5)
-    ap.add_argument("--load", type=float, default=0.85)
+    ap.add_argument("--load", type=float, default=0.9)
     a = ap.parse_args()
     print(json.dumps(serve_benchmark(a.requests, a.seed, a.load)))
