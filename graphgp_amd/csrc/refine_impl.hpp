@@ -134,6 +134,35 @@ __device__ __forceinline__ double bcast(double x) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// Sum over a P-lane group, the butterfly x += x[lane ^ o] for o = 1, 2, 4, ...  Offsets 1
+// and 2 stay inside the lane's quad and are DPP quad permutations ([1,0,3,2], [2,3,0,1]),
+// with no LDS round trip; larger offsets use the cross-lane permute.  Same additions in
+// the same order as the plain shuffle butterfly.
+template <int O>
+__device__ __forceinline__ int xor_quad_i32(int x) {
+  static_assert(O == 1 || O == 2, "quad offsets only");
+  return __builtin_amdgcn_update_dpp(0, x, O == 1 ? 0xB1 : 0x4E, 0xF, 0xF, true);
+}
+template <int O>
+__device__ __forceinline__ float xor_quad(float x) {
+  return __int_as_float(xor_quad_i32<O>(__float_as_int(x)));
+}
+template <int O>
+__device__ __forceinline__ double xor_quad(double x) {
+  long long v = __double_as_longlong(x);
+  int lo = xor_quad_i32<O>((int)(v & 0xffffffffll));
+  int hi = xor_quad_i32<O>((int)(v >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <int P, typename T>
+__device__ __forceinline__ T group_sum(T x) {
+  if constexpr (P >= 2) x += xor_quad<1>(x);
+  if constexpr (P >= 4) x += xor_quad<2>(x);
+#pragma unroll
+  for (int o = 4; o < P; o <<= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
 template <typename T>
 struct Sentinel;
 template <>
@@ -533,8 +562,7 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
             const int row = P * r + lane;
             if (row > i && row <= k) part += m[P * r * (r + 1) / 2 + i] * wv[r];
           }
-#pragma unroll
-          for (int o = 1; o < P; o <<= 1) part += __shfl_xor(part, o, 64);
+          part = group_sum<P>(part);
           const T wi = -part * bcast<P, il>(rdl[is]);
           if (lane == il) wv[is] = wi;
         }
@@ -575,9 +603,7 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
 #pragma unroll
       for (int r = 0; r < R; ++r)
         if (P * r + lane < k) part += wgrp[P * r + lane] * (double)uv[r];
-#pragma unroll
-      for (int o = 1; o < P; o <<= 1) part += __shfl_xor(part, o, 64);
-      return part;
+      return group_sum<P>(part);
     };
 
     // ---- forward substitution of the (transformed) values; returns row k's residual
