@@ -105,7 +105,10 @@ __device__ __forceinline__ void static_for(F&& f) {
 
 // Broadcast lane S of every P-lane group to the whole group.  The quad_perm sources are
 // always inside the lane's own quad, so no lane reads an invalid one; with bound_ctrl set
-// the DPP move needs no old value, and no copy into its destination first.
+// the DPP move needs no old value, and no copy into its destination first.  For 8-lane
+// groups a second DPP move copies the source quad's value into the group's other quad:
+// row_shr:4 into banks 1 and 3 (lanes 4-7 and 12-15 of each row read lanes 0-3 and 8-11),
+// or row_shl:4 into banks 0 and 2; the lanes of the other banks keep the first move's value.
 template <int P, int S>
 __device__ __forceinline__ int bcast_i32(int x) {
   if constexpr (P == 1) {
@@ -116,7 +119,13 @@ __device__ __forceinline__ int bcast_i32(int x) {
   } else if constexpr (P == 4) {
     return __builtin_amdgcn_update_dpp(0, x, S * 0x55, 0xF, 0xF, true);
   } else if constexpr (P == 8) {
-    return __builtin_amdgcn_ds_swizzle(x, 0x18 | (S << 5));
+    if constexpr (S < 4) {
+      const int t = __builtin_amdgcn_update_dpp(0, x, S * 0x55, 0xF, 0xF, true);
+      return __builtin_amdgcn_update_dpp(t, t, 0x114, 0xF, 0xA, true);  // row_shr:4, banks 1, 3
+    } else {
+      const int t = __builtin_amdgcn_update_dpp(0, x, (S - 4) * 0x55, 0xF, 0xF, true);
+      return __builtin_amdgcn_update_dpp(t, t, 0x104, 0xF, 0x5, true);  // row_shl:4, banks 0, 2
+    }
   } else {
     static_assert(P == 16, "P must be 1, 2, 4, 8 or 16");
     return __builtin_amdgcn_ds_swizzle(x, 0x10 | (S << 5));
