@@ -237,8 +237,7 @@ __global__ __launch_bounds__(256, 2) void adjoint_kernel(AdjointArgs<T> a) {
 #pragma unroll
     for (int r = 0; r < R; ++r)
       if (P * r + lane < k) bw += (double)bv[r] * (double)w[r];
-#pragma unroll
-    for (int o = 1; o < P; o <<= 1) bw += __shfl_xor(bw, o, 64);
+    bw = group_sum<P>(bw);
     const double c0 = (double)lookup2(T(0));
     const double sigma = sqrt(c0 - bw);
     const double xi = valid ? load_as<double>(a.xi, a.xi_f64, a.perm ? (int64_t)a.perm[fine] : fine) : 0.0;

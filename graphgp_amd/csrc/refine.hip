@@ -12,8 +12,8 @@
 //  * P lanes cooperate on one point; row i of the (k+1)^2 system lives in lane i % P,
 //    slot i / P, holding columns 0..P*slot+P-1 (VGPRs, compile-time indexed).
 //  * Matrix entries are produced on the fly: the neighbour coordinates are gathered
-//    once (one row per lane) and broadcast across the P lanes with DPP quad_perm
-//    (P <= 4) or ds_swizzle (P = 8); the covariance table sits in LDS as
+//    once (one row per lane) and broadcast across the P lanes with DPP moves (quad_perm,
+//    plus a row shift for P = 8, row_newbcast for P = 16); the covariance table sits in LDS as
 //    {x_j, f_j, slope_j, x_j+1} segments and r is located by a log2 guess plus an
 //    exact fix-up (binary search when the bins are not log spaced).
 //  * Right-looking Cholesky on the row-cyclic layout; the forward substitution of the

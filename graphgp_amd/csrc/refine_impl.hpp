@@ -109,6 +109,7 @@ __device__ __forceinline__ void static_for(F&& f) {
 // groups a second DPP move copies the source quad's value into the group's other quad:
 // row_shr:4 into banks 1 and 3 (lanes 4-7 and 12-15 of each row read lanes 0-3 and 8-11),
 // or row_shl:4 into banks 0 and 2; the lanes of the other banks keep the first move's value.
+// A 16-lane group is a DPP row: row_newbcast copies its lane S to the row.
 template <int P, int S>
 __device__ __forceinline__ int bcast_i32(int x) {
   if constexpr (P == 1) {
@@ -128,7 +129,7 @@ __device__ __forceinline__ int bcast_i32(int x) {
     }
   } else {
     static_assert(P == 16, "P must be 1, 2, 4, 8 or 16");
-    return __builtin_amdgcn_ds_swizzle(x, 0x10 | (S << 5));
+    return __builtin_amdgcn_update_dpp(0, x, 0x150 + S, 0xF, 0xF, true);  // row_newbcast:S
   }
 }
 template <int P, int S>
