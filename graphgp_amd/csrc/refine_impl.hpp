@@ -144,32 +144,41 @@ __device__ __forceinline__ double bcast(double x) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Sum over a P-lane group, the butterfly x += x[lane ^ o] for o = 1, 2, 4, ...  Offsets 1
-// and 2 stay inside the lane's quad and are DPP quad permutations ([1,0,3,2], [2,3,0,1]),
-// with no LDS round trip; larger offsets use the cross-lane permute.  Same additions in
+// Sum over a P-lane group, the butterfly x += x[lane ^ o] for o = 1, 2, 4, 8, all DPP moves
+// inside a 16-lane row, with no LDS round trip: offsets 1 and 2 are quad permutations
+// ([1,0,3,2], [2,3,0,1]); offset 4 is row_shl:4 into banks 0 and 2 (lanes read lane + 4)
+// and row_shr:4 into banks 1 and 3 (lane - 4); offset 8 is row_ror:8.  Same additions in
 // the same order as the plain shuffle butterfly.
 template <int O>
-__device__ __forceinline__ int xor_quad_i32(int x) {
-  static_assert(O == 1 || O == 2, "quad offsets only");
-  return __builtin_amdgcn_update_dpp(0, x, O == 1 ? 0xB1 : 0x4E, 0xF, 0xF, true);
+__device__ __forceinline__ int xor_lane_i32(int x) {
+  if constexpr (O == 1 || O == 2) {
+    return __builtin_amdgcn_update_dpp(0, x, O == 1 ? 0xB1 : 0x4E, 0xF, 0xF, true);
+  } else if constexpr (O == 4) {
+    const int t = __builtin_amdgcn_update_dpp(0, x, 0x104, 0xF, 0x5, true);
+    return __builtin_amdgcn_update_dpp(t, x, 0x114, 0xF, 0xA, true);
+  } else {
+    static_assert(O == 8, "offsets inside a 16-lane row only");
+    return __builtin_amdgcn_update_dpp(0, x, 0x128, 0xF, 0xF, true);
+  }
 }
 template <int O>
-__device__ __forceinline__ float xor_quad(float x) {
-  return __int_as_float(xor_quad_i32<O>(__float_as_int(x)));
+__device__ __forceinline__ float xor_lane(float x) {
+  return __int_as_float(xor_lane_i32<O>(__float_as_int(x)));
 }
 template <int O>
-__device__ __forceinline__ double xor_quad(double x) {
+__device__ __forceinline__ double xor_lane(double x) {
   long long v = __double_as_longlong(x);
-  int lo = xor_quad_i32<O>((int)(v & 0xffffffffll));
-  int hi = xor_quad_i32<O>((int)(v >> 32));
+  int lo = xor_lane_i32<O>((int)(v & 0xffffffffll));
+  int hi = xor_lane_i32<O>((int)(v >> 32));
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 template <int P, typename T>
 __device__ __forceinline__ T group_sum(T x) {
-  if constexpr (P >= 2) x += xor_quad<1>(x);
-  if constexpr (P >= 4) x += xor_quad<2>(x);
-#pragma unroll
-  for (int o = 4; o < P; o <<= 1) x += __shfl_xor(x, o, 64);
+  static_assert(P >= 1 && P <= 16, "groups of at most one DPP row");
+  if constexpr (P >= 2) x += xor_lane<1>(x);
+  if constexpr (P >= 4) x += xor_lane<2>(x);
+  if constexpr (P >= 8) x += xor_lane<4>(x);
+  if constexpr (P >= 16) x += xor_lane<8>(x);
   return x;
 }
 
