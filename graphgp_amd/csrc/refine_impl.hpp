@@ -843,12 +843,12 @@ __global__ __launch_bounds__(256) void apply_kernel(RefineArgs<T> a) {
             double part = 0.0;
 #pragma unroll
             for (int r = 0; r < RM; ++r) part += wd[r] * (double)u[r];
-#pragma unroll
-            for (int o = 1; o < P; o <<= 1) part += __shfl_xor(part, o, 64);
+            // a group's lanes finish together, so the group sum's DPP sources are active
+            part = group_sum<P>(part);
             v = (T)(part + a.sig64[slot] * (double)xi);
           } else {
             // neighbour 0 is lane 0's first slot
-            const T v0 = incr ? __shfl(u[0], gbase, 64) : T(0);
+            const T v0 = incr ? bcast<P, 0>(u[0]) : T(0);
             T part = T(0);
 #pragma unroll
             for (int r = 0; r < RM; ++r) {
@@ -856,8 +856,7 @@ __global__ __launch_bounds__(256) void apply_kernel(RefineArgs<T> a) {
               const T uj = (incr && j != 0) ? u[r] - v0 : u[r];
               part += (T)wd[r] * uj;
             }
-#pragma unroll
-            for (int o = 1; o < P; o <<= 1) part += __shfl_xor(part, o, 64);
+            part = group_sum<P>(part);
             v = v0 + part + sg * xi;
           }
           if (owner) {
