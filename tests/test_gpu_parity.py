@@ -299,6 +299,31 @@ def test_approaches_dense_generic_k():
     assert np.allclose(v, O.generate(og, ocov, xi), rtol=1e-9, atol=1e-10)
 
 
+# ---------------------------------------------------------------- every register configuration
+# One case per register configuration of the refinement kernel: k + 1 = 4, 8, ..., 40 runs
+# one, two, four and eight lanes per point in FP32 and in FP64, each with its own lane
+# broadcasts and group sums.
+@pytest.mark.parametrize("k", [3, 7, 11, 15, 19, 23, 27, 31, 39])
+def test_generate_each_register_configuration_vs_oracle(k):
+    rng = np.random.default_rng(k)
+    n = 3000
+    p = rng.uniform(0, 1, (n, 3)).astype(np.float32)
+    g = gp.build_graph(p, n0=max(k, 64), k=k)
+    og = O.OGraph(cpu(g.points), cpu(g.neighbors), g.offsets, cpu(g.indices))
+    kw = dict(p=1, variance=1.0, cutoff=0.2, r_min=1e-5, r_max=2.0, n_bins=2001, jitter=1e-5)
+    cov, ocov = gp.extras.matern_kernel(**kw), O.matern_kernel(**kw)
+    xi = rng.standard_normal(n)
+    ov = O.generate(og, ocov, xi)
+    scale = float(np.abs(ov).max())
+    v64 = cpu(gp.generate(g, cov, xi, precision="fp64"))
+    assert np.max(np.abs(v64 - ov)) <= 1e-9 * scale, (k, float(np.max(np.abs(v64 - ov))))
+    for prec, tol in (("fp32", 2e-2), ("fp32_increment", 2e-3)):
+        v = cpu(gp.generate(g, cov, xi.astype(np.float32), precision=prec))
+        assert np.max(np.abs(v - ov)) <= tol * scale, (k, prec, float(np.max(np.abs(v - ov))))
+    xb = cpu(gp.generate_inv(g, cov, ov, precision="fp64"))
+    assert np.max(np.abs(xb - xi)) <= 1e-6, (k, float(np.max(np.abs(xb - xi))))
+
+
 # ---------------------------------------------------------------- refine (FP32)
 @pytest.fixture(scope="module")
 def c1_setup():
