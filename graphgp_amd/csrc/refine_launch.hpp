@@ -46,6 +46,10 @@ int launch_config(RefineArgs<T> a, hipStream_t s, size_t lds_bytes, int grid) {
     GGP_TRY((launch_variant<T, KK, P, D, true, kWeights>(a, s, lds_bytes, grid)));
     return launch_variant<T, KK, P, D, false, kWeights>(a, s, lds_bytes, grid);
   }
+  if (a.mode == kGenerate) {  // the fused generate, likewise (fewer live values, fewer spills)
+    GGP_TRY((launch_variant<T, KK, P, D, true, kGenerate>(a, s, lds_bytes, grid)));
+    return launch_variant<T, KK, P, D, false, kGenerate>(a, s, lds_bytes, grid);
+  }
   GGP_TRY((launch_variant<T, KK, P, D, true, -1>(a, s, lds_bytes, grid)));
   return launch_variant<T, KK, P, D, false, -1>(a, s, lds_bytes, grid);
 }
