@@ -42,16 +42,25 @@ int launch_config(RefineArgs<T> a, hipStream_t s, size_t lds_bytes, int grid) {
     set_error("refine: covariance table does not fit in LDS (" + std::to_string(lds_bytes) + " bytes)");
     return kUnsupported;
   }
-  if (a.mode == kWeights) {  // the weights phases: an instantiation without the other modes' code
-    GGP_TRY((launch_variant<T, KK, P, D, true, kWeights>(a, s, lds_bytes, grid)));
-    return launch_variant<T, KK, P, D, false, kWeights>(a, s, lds_bytes, grid);
+  // one instantiation per mode, without the other modes' code (fewer live values, fewer
+  // spills than one kernel that reads the mode at run time)
+  switch (a.mode) {
+    case kWeights:
+      GGP_TRY((launch_variant<T, KK, P, D, true, kWeights>(a, s, lds_bytes, grid)));
+      return launch_variant<T, KK, P, D, false, kWeights>(a, s, lds_bytes, grid);
+    case kGenerate:
+      GGP_TRY((launch_variant<T, KK, P, D, true, kGenerate>(a, s, lds_bytes, grid)));
+      return launch_variant<T, KK, P, D, false, kGenerate>(a, s, lds_bytes, grid);
+    case kInverse:
+      GGP_TRY((launch_variant<T, KK, P, D, true, kInverse>(a, s, lds_bytes, grid)));
+      return launch_variant<T, KK, P, D, false, kInverse>(a, s, lds_bytes, grid);
+    case kLogdet:
+      GGP_TRY((launch_variant<T, KK, P, D, true, kLogdet>(a, s, lds_bytes, grid)));
+      return launch_variant<T, KK, P, D, false, kLogdet>(a, s, lds_bytes, grid);
+    default:
+      set_error("refine: unknown mode " + std::to_string(a.mode));
+      return kInvalidArgument;
   }
-  if (a.mode == kGenerate) {  // the fused generate, likewise (fewer live values, fewer spills)
-    GGP_TRY((launch_variant<T, KK, P, D, true, kGenerate>(a, s, lds_bytes, grid)));
-    return launch_variant<T, KK, P, D, false, kGenerate>(a, s, lds_bytes, grid);
-  }
-  GGP_TRY((launch_variant<T, KK, P, D, true, -1>(a, s, lds_bytes, grid)));
-  return launch_variant<T, KK, P, D, false, -1>(a, s, lds_bytes, grid);
 }
 
 
