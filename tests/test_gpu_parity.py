@@ -371,6 +371,22 @@ def test_nan_rule_non_pd():
     assert np.all(np.isnan(x))
 
 
+def test_indefinite_seed_block_nan_rows_match_oracle():
+    # the 1000 seeds of a 4,194,304-point clustered graph (tests/golden/make_indefinite_seeds.py);
+    # with jitter 1e-5 their interpolated covariance is indefinite, and the seed block's
+    # factorisation turns the same rows NaN as the oracle's
+    import os
+
+    p = np.load(os.path.join(os.path.dirname(__file__), "golden", "indefinite_seeds_4m.npy"))
+    kw = dict(p=1, variance=1.0, cutoff=0.05, r_min=1e-6, r_max=2.0, n_bins=2001, jitter=1e-5)
+    cov, ocov = gp.extras.matern_kernel(**kw), O.matern_kernel(**kw)
+    xi = np.ones(len(p))
+    ov = O.generate_dense(p, ocov, xi)
+    v = cpu(gp.generate_dense(p, cov, xi))
+    assert np.isnan(ov).any()
+    assert np.array_equal(np.isnan(v), np.isnan(ov)), (int(np.isnan(v).sum()), int(np.isnan(ov).sum()))
+
+
 def test_argument_errors():
     p = clustered_points(500, d=2, seed=3)
     g = gp.build_graph(p, n0=20, k=5)

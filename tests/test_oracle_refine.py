@@ -216,3 +216,16 @@ def test_generate_vjp_composition_pinned():
         e[j] = 1e-7
         fd = (loss(cov[1] + e) - loss(cov[1] - e)) / 2e-7
         assert np.isclose(gv[j], fd, rtol=1e-4, atol=1e-6)
+
+
+def test_indefinite_seed_fixture_is_indefinite():
+    # pins the fixture of the GPU test of the NaN rule: the interpolated covariance of these
+    # seeds has a negative eigenvalue, and the oracle's dense block turns some rows NaN
+    import os
+
+    p = np.load(os.path.join(os.path.dirname(__file__), "golden", "indefinite_seeds_4m.npy"))
+    bins, vals = O.matern_kernel(p=1, variance=1.0, cutoff=0.05, r_min=1e-6, r_max=2.0, n_bins=2001, jitter=1e-5)
+    P = p.astype(np.float64)
+    K = np.interp(np.linalg.norm(P[:, None] - P[None], axis=-1), bins, vals)
+    assert np.linalg.eigvalsh(K)[0] < 0
+    assert np.isnan(O.generate_dense(p, (bins, vals), np.ones(len(p)))).any()
