@@ -35,9 +35,12 @@ __device__ inline double interp_dev(double x, const double* __restrict__ xp, con
   return fp[i - 1] + (delta / dx) * df;
 }
 
+// With `gate` non-null the kernel runs only when *gate is not finite (the log-det
+// fallback below rebuilds K after a failed Cholesky).
 __global__ void dense_cov_kernel(const void* __restrict__ pts, int pts_f64, int d, int64_t n,
                                  const double* __restrict__ bins, const double* __restrict__ vals, int nb,
-                                 double* __restrict__ A) {
+                                 double* __restrict__ A, const double* __restrict__ gate) {
+  if (gate != nullptr && isfinite(*gate)) return;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * n; e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = e / n, j = e - i * n;
     double s = 0.0;
@@ -332,6 +335,97 @@ __global__ __launch_bounds__(1024) void logdiag_kernel(const double* __restrict_
   }
 }
 
+// log|det K| / 2 by Gaussian elimination with partial pivoting (the first row of largest
+// |a_ij| in the column, LAPACK getrf's rule), which is what jnp.linalg.slogdet computes
+// (refine.py:248): finite for an indefinite K, -inf for a singular one, NaN for NaN input.
+// Runs only when *logdet (the Cholesky's sum of log L_jj) is not finite, i.e. when the
+// Cholesky failed; then K has been rebuilt in A by the gated dense_cov_kernel.  One
+// workgroup: per column a pivot search, a row swap (columns >= j only; the multipliers are
+// not kept) and the trailing update, each wavefront taking rows and its lanes columns.
+// Column j and row j are only read during the update, so no staging is needed.
+__global__ __launch_bounds__(1024) void lu_logdet_kernel(double* __restrict__ A, int64_t n, double* __restrict__ logdet) {
+  if (isfinite(*logdet)) return;
+  __shared__ double s_best[16];
+  __shared__ int64_t s_arg[16];
+  __shared__ int64_t s_piv;
+  __shared__ int s_stop;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  double acc = 0.0;  // thread 0 only
+  for (int64_t j = 0; j < n; ++j) {
+    double best = -1.0;
+    int64_t arg = j;
+    for (int64_t i = j + t; i < n; i += 1024) {
+      const double a = fabs(A[i * n + j]);
+      if (a != a) {
+        best = a;  // NaN wins the search and stops the elimination
+        arg = i;
+        break;
+      }
+      if (a > best) {
+        best = a;
+        arg = i;
+      }
+    }
+    // reduction: NaN first, then the largest, then the lowest row
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ob = __shfl_xor(best, o, 64);
+      const int64_t oa = __shfl_xor(arg, o, 64);
+      const bool take = (ob != ob) ? !(best != best && arg < oa) : (best == best && (ob > best || (ob == best && oa < arg)));
+      if (take) {
+        best = ob;
+        arg = oa;
+      }
+    }
+    if (lane == 0) {
+      s_best[w] = best;
+      s_arg[w] = arg;
+    }
+    __syncthreads();
+    if (t == 0) {
+      double b = s_best[0];
+      int64_t a = s_arg[0];
+      for (int q = 1; q < 16; ++q) {
+        const double ob = s_best[q];
+        const int64_t oa = s_arg[q];
+        const bool take = (ob != ob) ? !(b != b && a < oa) : (b == b && (ob > b || (ob == b && oa < a)));
+        if (take) {
+          b = ob;
+          a = oa;
+        }
+      }
+      int stop = 0;
+      if (b != b) {
+        acc = __builtin_nan("");
+        stop = 1;
+      } else if (b == 0.0) {
+        acc = -__builtin_inf();
+        stop = 1;
+      }
+      s_piv = a;
+      s_stop = stop;
+    }
+    __syncthreads();
+    if (s_stop) break;
+    const int64_t p = s_piv;
+    if (p != j)
+      for (int64_t c = j + t; c < n; c += 1024) {
+        const double x = A[j * n + c];
+        A[j * n + c] = A[p * n + c];
+        A[p * n + c] = x;
+      }
+    __syncthreads();
+    const double piv = A[j * n + j];
+    if (t == 0) acc += log(fabs(piv));
+    for (int64_t i = j + 1 + w; i < n; i += 16) {
+      const double l = A[i * n + j] / piv;
+      if (l == 0.0) continue;
+      for (int64_t c = j + 1 + lane; c < n; c += 64) A[i * n + c] -= l * A[j * n + c];
+    }
+    __syncthreads();
+  }
+  if (t == 0) *logdet = 0.5 * acc;
+}
+
 }  // namespace
 
 size_t dense_workspace(int64_t n0) { return (size_t)(n0 * n0 + NB * NB) * sizeof(double) + 256; }
@@ -340,7 +434,7 @@ int dense_factor(const void* pts, int pts_f64, int d, int64_t n0, const double* 
                  double* A, hipStream_t s) {
   if (n0 <= 0) return kOk;
   hipLaunchKernelGGL(dense_cov_kernel, dim3(grid_capped(n0 * n0, 256)), dim3(256), 0, s, pts, pts_f64, d, n0, bins,
-                     vals, nb, A);
+                     vals, nb, A, (const double*)nullptr);
   GGP_LAUNCH_CHECK();
   const int nblk = (int)((n0 + NB - 1) / NB);
   double* Linv = A + (size_t)n0 * n0;  // 64 x 64 scratch after the matrix
@@ -380,6 +474,17 @@ int cov_matrix(const void* a, int64_t na, const void* b, int64_t nb_, int d, int
 
 int dense_logdiag(const double* L, int64_t n0, double* out, hipStream_t s) {
   hipLaunchKernelGGL(logdiag_kernel, dim3(1), dim3(1024), 0, s, L, n0, out);
+  GGP_LAUNCH_CHECK();
+  return kOk;
+}
+
+int dense_logdet_fallback(const void* pts, int pts_f64, int d, int64_t n0, const double* bins, const double* vals,
+                          int nb, double* A, double* logdet, hipStream_t s) {
+  if (n0 <= 0) return kOk;
+  hipLaunchKernelGGL(dense_cov_kernel, dim3(grid_capped(n0 * n0, 256)), dim3(256), 0, s, pts, pts_f64, d, n0, bins,
+                     vals, nb, A, (const double*)logdet);
+  GGP_LAUNCH_CHECK();
+  hipLaunchKernelGGL(lu_logdet_kernel, dim3(1), dim3(1024), 0, s, A, n0, logdet);
   GGP_LAUNCH_CHECK();
   return kOk;
 }

@@ -12,4 +12,9 @@ int dense_solve(const double* L, int64_t n0, double* v, hipStream_t s);         
 int cov_matrix(const void* a, int64_t na, const void* b, int64_t nb, int d, int f64, const double* bins,
                const double* vals, int nbins, double* out, hipStream_t s);
 int dense_logdiag(const double* L, int64_t n0, double* out, hipStream_t s);                 // sum log L_ii
+// When *logdet (from dense_logdiag) is not finite -- the Cholesky failed -- rebuild K in A
+// and overwrite *logdet with log|det K| / 2 from a partially pivoted LU (slogdet's value);
+// otherwise both kernels return at once.
+int dense_logdet_fallback(const void* pts, int pts_f64, int d, int64_t n0, const double* bins, const double* vals,
+                          int nb, double* A, double* logdet, hipStream_t s);
 }  // namespace ggp

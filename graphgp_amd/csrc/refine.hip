@@ -410,7 +410,7 @@ bool carve(Workspace& w, const RefineProblem& p, int mode, RefineWs<T>& b) {
   }
   const int64_t n0 = p.n - p.n_ref;
   b.partials = w.take<double>((p.n_ref + 31) / 32 + 1);
-  if (mode != kLogdet || p.with_seed) b.vals = w.take<T>(p.n);
+  if (mode != kLogdet) b.vals = w.take<T>(p.n);  // the log-det reads no values
   if (p.with_seed) {
     b.dense = w.take<double>((size_t)n0 * n0 + 4096);
     b.dvec = w.take<double>(n0 + 1);
@@ -671,6 +671,7 @@ int refine_flow(const RefineProblem& p, int mode, const RefineIO& io, void* ws, 
   if (p.with_seed && n0 > 0) {
     GGP_TRY(dense_factor(p.pts, p.pts_f64, p.d, n0, p.bins, p.vals, p.nb, b.dense, s));
     GGP_TRY(dense_logdiag(b.dense, n0, b.dout, s));
+    GGP_TRY(dense_logdet_fallback(p.pts, p.pts_f64, p.d, n0, p.bins, p.vals, p.nb, b.dense, b.dout, s));
     seed = b.dout;
   }
   hipLaunchKernelGGL(logdet_sum_kernel, dim3(1), dim3(1024), 0, s, b.partials, ntiles, seed, (double*)io.out);
@@ -740,7 +741,10 @@ int dense_flow(const void* pts, int pts_f64, int d, int64_t n0, const double* bi
   }
   GGP_TRY(fill_u32(flags, 0u, 4, s));
   GGP_TRY(dense_factor(pts, pts_f64, d, n0, bins, vals, nb, A, s));
-  if (mode == kLogdet) return dense_logdiag(A, n0, (double*)out, s);
+  if (mode == kLogdet) {
+    GGP_TRY(dense_logdiag(A, n0, (double*)out, s));
+    return dense_logdet_fallback(pts, pts_f64, d, n0, bins, vals, nb, A, (double*)out, s);
+  }
   hipLaunchKernelGGL(gather_seed_xi_kernel, dim3(grid_for(n0, 256)), dim3(256), 0, s, in, in_f64,
                      (const int32_t*)nullptr, n0, v);
   GGP_LAUNCH_CHECK();
@@ -752,6 +756,11 @@ int dense_flow(const void* pts, int pts_f64, int d, int64_t n0, const double* bi
   }
   hipLaunchKernelGGL(seed_out_kernel<double>, dim3(grid_for(n0, 256)), dim3(256), 0, s, o, (const int32_t*)nullptr,
                      n0, (double*)nullptr, out, out_f64, flags);
+  GGP_LAUNCH_CHECK();
+  // a failed factorisation (any NaN) makes the whole output NaN, as jnp.linalg.cholesky's
+  // all-NaN factor does in the reference (refine.py:74, :193)
+  hipLaunchKernelGGL(nan_fill_kernel, dim3(grid_for(std::min<int64_t>(n0, 1 << 20), 256)), dim3(256), 0, s, out,
+                     out_f64, n0, flags);
   GGP_LAUNCH_CHECK();
   return kOk;
 }

@@ -71,14 +71,15 @@ SCHEDULE_MIN_REFINED = 1 << 20
 
 
 def _schedule(graph: Graph, pts, nb):
-    """The graph's dataflow schedule, computed on first use and kept on the graph (its
-    arrays are treated as immutable, like the reference's JAX arrays); None for small
-    graphs or more than 65535 batches."""
+    """The graph's dataflow schedule, computed on first use and kept on the graph; None for
+    small graphs or more than 65535 batches.  The cache key holds the tensors' addresses
+    and version counters, so a graph whose points or neighbours were changed in place
+    gets a fresh schedule instead of stale slot-order copies."""
     n, d = pts.shape
     n_ref, k = nb.shape
     if n_ref < SCHEDULE_MIN_REFINED or len(graph.offsets) > 65536:
         return None
-    key = (pts.data_ptr(), nb.data_ptr(), tuple(graph.offsets))
+    key = (pts.data_ptr(), pts._version, nb.data_ptr(), nb._version, tuple(graph.offsets))
     cached = getattr(graph, "_ggp_schedule", None)
     if cached is not None and cached[0] in (key, "off"):
         return cached[1]

@@ -214,12 +214,18 @@ inline double pdist(const double* a, const double* b, int d) {
   return std::sqrt(s);
 }
 
-// In-place lower Cholesky of an n x n row-major matrix (jnp.linalg.cholesky).
-// Failure propagates NaN, as XLA's cholesky does.
+// In-place lower Cholesky of an n x n row-major matrix (jnp.linalg.cholesky).  On failure
+// (a pivot that is not positive, or NaN input) the whole factor is NaN: JAX's CPU cholesky
+// replaces the factor by NaN when LAPACK's potrf reports an error.
 void cholesky(double* A, int n) {
-  for (int j = 0; j < n; ++j) {
+  bool failed = false;
+  for (int j = 0; j < n && !failed; ++j) {
     double s = A[j * n + j];
     for (int m = 0; m < j; ++m) s -= A[j * n + m] * A[j * n + m];
+    if (!(s > 0.0)) {
+      failed = true;
+      break;
+    }
     double djj = std::sqrt(s);
     A[j * n + j] = djj;
     for (int i = j + 1; i < n; ++i) {
@@ -229,6 +235,39 @@ void cholesky(double* A, int n) {
     }
     for (int i = 0; i < j; ++i) A[i * n + j] = 0.0;
   }
+  if (failed)
+    for (int64_t e = 0; e < (int64_t)n * n; ++e) A[e] = std::numeric_limits<double>::quiet_NaN();
+}
+
+// log|det A| of an n x n row-major matrix by Gaussian elimination with partial pivoting
+// (LAPACK getrf's rule: the first row of largest |a_ij| in the column), which is what
+// jnp.linalg.slogdet computes: the sum of log|u_jj|, -inf for a singular matrix, NaN for
+// NaN input.  A is overwritten.
+double lu_logabsdet(double* A, int64_t n) {
+  double s = 0.0;
+  for (int64_t j = 0; j < n; ++j) {
+    int64_t p = j;
+    double best = std::fabs(A[j * n + j]);
+    for (int64_t i = j + 1; i < n; ++i) {
+      const double a = std::fabs(A[i * n + j]);
+      if (a > best) {
+        best = a;
+        p = i;
+      }
+    }
+    if (best != best) return std::numeric_limits<double>::quiet_NaN();
+    if (best == 0.0) return -std::numeric_limits<double>::infinity();
+    if (p != j)
+      for (int64_t c = 0; c < n; ++c) std::swap(A[j * n + c], A[p * n + c]);
+    const double piv = A[j * n + j];
+    s += std::log(std::fabs(piv));
+    for (int64_t i = j + 1; i < n; ++i) {
+      const double l = A[i * n + j] / piv;
+      if (l == 0.0) continue;
+      for (int64_t c = j + 1; c < n; ++c) A[i * n + c] -= l * A[j * n + c];
+    }
+  }
+  return s;
 }
 
 struct Table { const double* bins; const double* vals; int n; };
@@ -391,15 +430,13 @@ void ora_generate_dense_inv(const double* pts, int64_t n, int d, const double* b
   }
 }
 
-// refine.py:243-248 generate_dense_logdet = slogdet(K)[1] / 2 = sum log diag(chol K).
+// refine.py:243-248 generate_dense_logdet = slogdet(K)[1] / 2: log|det K| / 2 by the
+// partially pivoted LU that slogdet uses, so an indefinite K still gives a finite value.
 double ora_generate_dense_logdet(const double* pts, int64_t n, int d, const double* bins, const double* vals,
                                  int nbins) {
   std::vector<double> K(n * n);
   ora_cov_matrix(pts, n, pts, n, d, bins, vals, nbins, K.data());
-  cholesky(K.data(), (int)n);
-  double s = 0.0;
-  for (int64_t i = 0; i < n; ++i) s += std::log(K[i * n + i]);
-  return s;
+  return 0.5 * lu_logabsdet(K.data(), n);
 }
 
 // refine.py:128-135 for every refined point: mean_vec (n_ref x k) and std (n_ref).

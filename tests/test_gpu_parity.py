@@ -225,6 +225,53 @@ def test_check_graph_detects_errors():
         gp.check_graph(gp.Graph(g.points, bad, g.offsets, g.indices))
     with pytest.raises(AssertionError, match="first offset"):
         gp.check_graph(gp.Graph(g.points, g.neighbors, (49,) + g.offsets[1:], g.indices))
+    # graph.py:58-60: non-decreasing offsets, last offset within the points
+    off = list(g.offsets)
+    off[2], off[3] = off[3], off[2]
+    with pytest.raises(AssertionError, match="non-decreasing"):
+        gp.check_graph(gp.Graph(g.points, g.neighbors, tuple(off), g.indices))
+    with pytest.raises(AssertionError, match="Last offset"):
+        gp.check_graph(gp.Graph(g.points, g.neighbors, g.offsets[:-1] + (3001,), g.indices))
+    # graph.py:68-70: a neighbour inside its own batch (but preceding, so the order holds)
+    b = 3
+    i = g.offsets[b] + 5
+    assert i < g.offsets[b + 1]
+    bad = g.neighbors.clone()
+    bad[i - g.offsets[0], 0] = g.offsets[b]
+    with pytest.raises(AssertionError, match="same batch"):
+        gp.check_graph(gp.Graph(g.points, bad, g.offsets, g.indices))
+
+
+def test_check_graph_device_status_bits():
+    # the device rules through the C ABI, including the offsets bit that the Python layer
+    # asserts on the host before it launches (bit 0 malformed offsets, 1 order, 2 same batch)
+    import ctypes
+
+    from graphgp_amd import _lib
+
+    p = clustered_points(3000, d=2, seed=1)
+    g = gp.build_graph(p, n0=50, k=6)
+    n_ref, k = g.neighbors.shape
+
+    def status(nb, offsets):
+        off = torch.tensor(offsets, dtype=torch.int64, device="cuda")
+        st = torch.zeros(1, dtype=torch.int32, device="cuda")
+        _lib.check(_lib.lib().ggp_check_graph(_lib.ptr(nb), n_ref, k, 3000, _lib.ptr(off), len(offsets),
+                                              _lib.ptr(st), _lib.stream()), "check_graph")
+        return int(st.cpu()[0])
+
+    assert status(g.neighbors, g.offsets) == 0
+    off = list(g.offsets)
+    off[2], off[3] = off[3], off[2]
+    assert status(g.neighbors, off) & 1
+    assert status(g.neighbors, list(g.offsets[:-1]) + [3001]) & 1
+    assert status(g.neighbors, [49] + list(g.offsets[1:])) & 1
+    bad = g.neighbors.clone()
+    bad[-1, 0] = 2999
+    assert status(bad, g.offsets) & 2
+    bad = g.neighbors.clone()
+    bad[g.offsets[3] + 5 - 50, 0] = g.offsets[3]
+    assert status(bad, g.offsets) == 4
 
 
 # ---------------------------------------------------------------- refine (FP64)
@@ -371,20 +418,45 @@ def test_nan_rule_non_pd():
     assert np.all(np.isnan(x))
 
 
-def test_indefinite_seed_block_nan_rows_match_oracle():
+def test_indefinite_seed_block_reference_rules():
     # the 1000 seeds of a 4,194,304-point clustered graph (tests/golden/make_indefinite_seeds.py);
-    # with jitter 1e-5 their interpolated covariance is indefinite, and the seed block's
-    # factorisation turns the same rows NaN as the oracle's
+    # with jitter 1e-5 their interpolated covariance is indefinite.  As in the reference,
+    # the failed Cholesky makes every value of generate_dense / generate_dense_inv NaN
+    # (refine.py:74, :193), while the log-det is slogdet's finite log|det K| / 2 (refine.py:248)
     import os
 
     p = np.load(os.path.join(os.path.dirname(__file__), "golden", "indefinite_seeds_4m.npy"))
     kw = dict(p=1, variance=1.0, cutoff=0.05, r_min=1e-6, r_max=2.0, n_bins=2001, jitter=1e-5)
     cov, ocov = gp.extras.matern_kernel(**kw), O.matern_kernel(**kw)
     xi = np.ones(len(p))
-    ov = O.generate_dense(p, ocov, xi)
-    v = cpu(gp.generate_dense(p, cov, xi))
-    assert np.isnan(ov).any()
-    assert np.array_equal(np.isnan(v), np.isnan(ov)), (int(np.isnan(v).sum()), int(np.isnan(ov).sum()))
+    assert np.all(np.isnan(O.generate_dense(p, ocov, xi)))
+    assert np.all(np.isnan(cpu(gp.generate_dense(p, cov, xi))))
+    assert np.all(np.isnan(cpu(gp.generate_dense_inv(p, cov, xi))))
+    ld = float(gp.generate_dense_logdet(p, cov))
+    old = O.generate_dense_logdet(p, ocov)
+    assert np.isfinite(ld) and np.isfinite(old)
+    assert np.isclose(ld, old, rtol=1e-9), (ld, old)
+
+
+def test_logdet_with_indefinite_seed_block_is_finite():
+    # generate_logdet = seed block's slogdet / 2 + sum of log std: finite although the seed
+    # block is indefinite; generate and generate_inv are all NaN
+    import os
+
+    seeds = np.load(os.path.join(os.path.dirname(__file__), "golden", "indefinite_seeds_4m.npy"))
+    rng = np.random.default_rng(5)
+    extra = (seeds[rng.integers(0, len(seeds), 4000)] + rng.normal(0, 0.01, (4000, 3))).astype(np.float32)
+    kw = dict(p=1, variance=1.0, cutoff=0.05, r_min=1e-6, r_max=2.0, n_bins=2001, jitter=1e-5)
+    cov, ocov = gp.extras.matern_kernel(**kw), O.matern_kernel(**kw)
+    g = gp.build_graph(np.concatenate([seeds.astype(np.float32), extra]), n0=1000, k=10)
+    og = O.OGraph(cpu(g.points), cpu(g.neighbors), g.offsets, cpu(g.indices))
+    n0 = g.offsets[0]
+    sd = O.generate_dense_logdet(og.points[:n0], ocov)
+    if not np.isfinite(sd) or not np.all(np.isnan(O.generate_dense(og.points[:n0], ocov, np.ones(n0)))):
+        pytest.skip("the tree's first 1000 points form a positive-definite seed block")
+    ld = float(gp.generate_logdet(g, cov, precision="fp64"))
+    assert np.isclose(ld, O.generate_logdet(og, ocov), rtol=1e-9)
+    assert np.all(np.isnan(cpu(gp.generate(g, cov, np.ones(5000), precision="fp64"))))
 
 
 def test_argument_errors():
@@ -451,3 +523,11 @@ def test_scheduled_generate_equals_unscheduled(monkeypatch):
     for pr in ("fp32", "fp64"):
         v = gp.generate(g, cov, xi, precision=pr)
         assert torch.equal(v, ref[pr]), pr
+    # points changed in place: the cached schedule (slot-order copies) must not be reused
+    with torch.no_grad():
+        g.points[n0 + 7] += 1e-3
+        g.points[-1] -= 1e-3
+    a = gp.generate(g, cov, xi, precision="fp32_increment")
+    monkeypatch.setattr(R, "SCHEDULE_MIN_REFINED", 1 << 40)
+    b = gp.generate(g, cov, xi, precision="fp32_increment")
+    assert torch.equal(a.view(torch.int32), b.view(torch.int32))

@@ -229,3 +229,37 @@ def test_indefinite_seed_fixture_is_indefinite():
     K = np.interp(np.linalg.norm(P[:, None] - P[None], axis=-1), bins, vals)
     assert np.linalg.eigvalsh(K)[0] < 0
     assert np.isnan(O.generate_dense(p, (bins, vals), np.ones(len(p)))).any()
+
+
+def test_seed_block_failure_rules_on_indefinite_block():
+    # the 1000 seeds of a 4,194,304-point clustered graph (tests/golden/make_indefinite_seeds.py):
+    # with jitter 1e-5 the interpolated covariance is indefinite.  jnp.linalg.cholesky then
+    # returns an all-NaN factor (every value of generate_dense / generate_dense_inv is NaN),
+    # while slogdet(K)[1] / 2 stays finite: numpy's LAPACK slogdet pins the oracle's LU.
+    import os
+
+    p = np.load(os.path.join(os.path.dirname(__file__), "golden", "indefinite_seeds_4m.npy"))
+    cov = O.matern_kernel(p=1, variance=1.0, cutoff=0.05, r_min=1e-6, r_max=2.0, n_bins=2001, jitter=1e-5)
+    K = O.compute_cov_matrix(cov, p, p)
+    sign, logabs = np.linalg.slogdet(K)
+    assert sign == -1.0
+    assert np.isclose(O.generate_dense_logdet(p, cov), logabs / 2, rtol=1e-12)
+    assert np.all(np.isnan(O.generate_dense(p, cov, np.ones(len(p)))))
+    assert np.all(np.isnan(O.generate_dense_inv(p, cov, np.ones(len(p)))))
+
+
+@pytest.mark.parametrize("n", [1, 2, 17, 300])
+def test_dense_logdet_is_slogdet(n):
+    # positive-definite blocks: the LU restatement agrees with LAPACK's slogdet and with the
+    # Cholesky's log-diagonal; a singular block (duplicate points, no jitter) gives -inf
+    rng = np.random.default_rng(n)
+    p = rng.uniform(0, 1, (n, 3))
+    cov = O.rbf_kernel(variance=1.0, scale=0.2, r_min=1e-5, r_max=2.0, n_bins=1000, jitter=1e-4)
+    K = O.compute_cov_matrix(cov, p, p)
+    assert np.isclose(O.generate_dense_logdet(p, cov), np.linalg.slogdet(K)[1] / 2, rtol=1e-11, atol=1e-13)
+    assert np.isclose(O.generate_dense_logdet(p, cov), np.sum(np.log(np.diag(np.linalg.cholesky(K)))),
+                      rtol=1e-11, atol=1e-13)
+    if n >= 2:
+        q = np.concatenate([p, p[:1]])
+        cov0 = O.rbf_kernel(variance=1.0, scale=0.2, r_min=1e-5, r_max=2.0, n_bins=1000, jitter=0.0)
+        assert O.generate_dense_logdet(q, cov0) == -np.inf
