@@ -63,6 +63,7 @@ _SIGS = {
     "ggp_generate_dense": (_int, [_p, _int, _i64, _int, _p, _p, _int, _p, _int, _p, _int, _p, _c_size, _p]),
     "ggp_generate_dense_inv": (_int, [_p, _int, _i64, _int, _p, _p, _int, _p, _int, _p, _int, _p, _c_size, _p]),
     "ggp_generate_dense_logdet": (_int, [_p, _int, _i64, _int, _p, _p, _int, _p, _p, _c_size, _p]),
+    "ggp_generate_logdet_noalloc": (_int, [_p, _int, _i64, _int, _p, _i64, _int, _p, _p, _int, _p, _int, _p]),
     "ggp_cov_matrix": (_int, [_p, _i64, _p, _i64, _int, _int, _p, _p, _int, _p, _p]),
     "ggp_profile_events": (_int, [_p, _p]),
     "ggp_refine_counters": (_int, [_p, _u32p, _p]),

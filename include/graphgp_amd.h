@@ -196,6 +196,19 @@ int ggp_generate_dense_logdet(const void* points, int pdtype, int64_t n, int d, 
 int ggp_cov_matrix(const void* points_a, int64_t na, const void* points_b, int64_t nb, int d, int pdtype,
                    const double* cov_bins, const double* cov_vals, int n_bins, double* out, void* stream);
 
+/* ---- generate_logdet without a workspace (no counterpart in the reference; for callers that
+ * cannot allocate device memory: code under HIP graph capture, serving and fitting loops,
+ * harnesses that forbid hipMalloc).  Same arguments and result as ggp_generate_logdet, minus
+ * the workspace: its scratch is a static device array of GGP_NOALLOC_SCRATCH_BYTES per GPU,
+ * held until the process ends.  A problem whose log-det scratch exceeds that size
+ * (ggp_refine_workspace's log-det share: about 8 MB for n0 = 1000 plus N/4 bytes) is
+ * rejected with status -1.  Allocates nothing.  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (several streams or host threads) race. */
+#define GGP_NOALLOC_SCRATCH_BYTES (64u << 20)
+int ggp_generate_logdet_noalloc(const void* points, int pdtype, int64_t n, int d, const int32_t* neighbors,
+                                int64_t n0, int k, const double* cov_bins, const double* cov_vals, int n_bins,
+                                double* logdet_device, int precision, void* stream);
+
 /* ---- counters of the last refinement flow that used `workspace` (syncs): [0] tiles issued,
  * [1] NaN produced, [2] malformed graph, [3] points recomputed in FP64 by the
  * GGP_PREC_FP32_INC mode (no reference counterpart). */
