@@ -131,6 +131,93 @@ void build_tree(const T* pts_in, int64_t n, int d, T* pts_out, int32_t* split_ou
   for (int64_t i = 0; i < n; ++i) { split_out[i] = split[i]; idx_out[i] = indices[i]; }
 }
 
+// The same sort as build_tree above, decomposed.  Before each level's sort, every node's
+// members form one contiguous run of the array: the array is sorted by the previous
+// level's node ids, and _update_nodes (tree.py:211-234) splits a node's run into a left
+// part, its median and a right part.  Sorting by (node, value, array position) therefore
+// places each run, sorted by (value, position), at the offset given by the number of
+// members of the smaller node ids, and the runs can be sorted independently (in
+// parallel).  The split dimension per node is the same first argmax of the extents, over
+// the same members.  Results equal build_tree's bit for bit (tests compare both).
+template <typename T>
+void build_tree_runs(const T* pts_in, int64_t n, int d, T* pts_out, int32_t* split_out, int64_t* idx_out) {
+  std::vector<T> pts(pts_in, pts_in + n * d), tmp(n * d);
+  std::vector<int64_t> nodes(n, 0), indices(n), tmpi(n), tmpn(n), count(n + 1);
+  std::vector<int32_t> split(n, -1);
+  std::iota(indices.begin(), indices.end(), 0);
+  int levels = 0;
+  while ((n >> levels) != 0) ++levels;
+  const T inf = std::numeric_limits<T>::infinity();
+  std::vector<int64_t> run_start;
+  for (int level = 0; level < levels; ++level) {
+    const int64_t n_above = (int64_t(1) << level) - 1;
+    // runs of equal node id
+    run_start.clear();
+    for (int64_t e = 0; e < n; ++e)
+      if (e == 0 || nodes[e] != nodes[e - 1]) run_start.push_back(e);
+    const int64_t nruns = (int64_t)run_start.size();
+    run_start.push_back(n);
+    // nodes >= n_above without members get split 0 (empty extents); the others below
+#pragma omp parallel for schedule(static)
+    for (int64_t nd = n_above; nd < n; ++nd) split[nd] = 0;
+    std::fill(count.begin(), count.end(), 0);
+    for (int64_t r = 0; r < nruns; ++r) count[nodes[run_start[r]] + 1] += run_start[r + 1] - run_start[r];
+    for (int64_t nd = 0; nd < n; ++nd) count[nd + 1] += count[nd];  // count[nd] = first slot of node nd
+#pragma omp parallel
+    {
+      std::vector<std::pair<T, int64_t>> buf;
+#pragma omp for schedule(dynamic, 1)
+      for (int64_t r = 0; r < nruns; ++r) {
+        const int64_t a = run_start[r], b = run_start[r + 1], nd = nodes[a];
+        int best = 0;
+        if (nd >= n_above) {
+          T bestv = T(0);
+          for (int j = 0; j < d; ++j) {
+            T mx = -inf, mn = inf;
+            for (int64_t e = a; e < b; ++e) {
+              const T x = pts[e * d + j];
+              if (x > mx) mx = x;
+              if (x < mn) mn = x;
+            }
+            const T ext = mx - mn;
+            if (j == 0) {
+              bestv = ext;
+            } else if (ext > bestv) {
+              bestv = ext;
+              best = j;
+            }
+          }
+          split[nd] = best;
+        }
+        const int dim = split[nd] < 0 ? 0 : split[nd];
+        buf.resize(b - a);
+        for (int64_t e = a; e < b; ++e) buf[e - a] = {pts[e * d + dim], e};
+        if (b - a > 1)
+          std::sort(buf.begin(), buf.end(), [](const std::pair<T, int64_t>& x, const std::pair<T, int64_t>& y) {
+            if (!key_eq(x.first, y.first)) return key_less(x.first, y.first);
+            return x.second < y.second;
+          });
+        const int64_t dst = count[nd];
+        for (int64_t q = 0; q < b - a; ++q) {
+          const int64_t src = buf[q].second;
+          std::memcpy(&tmp[(dst + q) * d], &pts[src * d], sizeof(T) * d);
+          tmpi[dst + q] = indices[src];
+          tmpn[dst + q] = nd;
+        }
+      }
+    }
+    pts.swap(tmp);
+    indices.swap(tmpi);
+    nodes.swap(tmpn);
+    update_nodes(nodes, level);
+  }
+  std::memcpy(pts_out, pts.data(), sizeof(T) * n * d);
+  for (int64_t i = 0; i < n; ++i) {
+    split_out[i] = split[i];
+    idx_out[i] = indices[i];
+  }
+}
+
 template <typename T>
 inline T sqdist(const T* a, const T* b, int d) {
   T s = (a[0] - b[0]) * (a[0] - b[0]);
@@ -197,12 +284,20 @@ void query_one(const T* pts, const int32_t* split, int d, int64_t qi, int64_t ma
 inline double interp(double x, const double* xp, const double* fp, int n) {
   if (x < xp[0]) return fp[0];
   if (x > xp[n - 1]) return fp[n - 1];
-  int i = (int)(std::upper_bound(xp, xp + n, x) - xp);  // searchsorted side='right'
+  // searchsorted side='right' (the first i with xp[i] > x), as a binary search without
+  // data-dependent branches: the same index as std::upper_bound
+  const double* base = xp;
+  int len = n;
+  while (len > 1) {
+    const int half = len >> 1;
+    base = (base[half] <= x) ? base + half : base;
+    len -= half;
+  }
+  int i = (int)(base - xp) + (*base <= x ? 1 : 0);
   if (i < 1) i = 1;
   if (i > n - 1) i = n - 1;
   double df = fp[i] - fp[i - 1], dx = xp[i] - xp[i - 1], delta = x - xp[i - 1];
-  const double eps = std::nextafter(std::numeric_limits<double>::epsilon(), 1.0) -
-                     std::numeric_limits<double>::epsilon();  // np.spacing(eps)
+  constexpr double eps = 4.930380657631324e-32;  // np.spacing(np.finfo(float).eps)
   if (std::fabs(dx) <= eps) return fp[i - 1];
   return fp[i - 1] + (delta / dx) * df;
 }
@@ -321,9 +416,16 @@ void ora_set_threads(int n) {
 
 // ------------------------------------------------------------------ tree.py
 void ora_build_tree_f32(const float* p, int64_t n, int d, float* po, int32_t* s, int64_t* idx) {
-  build_tree<float>(p, n, d, po, s, idx);
+  build_tree_runs<float>(p, n, d, po, s, idx);
 }
 void ora_build_tree_f64(const double* p, int64_t n, int d, double* po, int32_t* s, int64_t* idx) {
+  build_tree_runs<double>(p, n, d, po, s, idx);
+}
+// the literal whole-array sort per level (tree.py:153-184), for pinning build_tree_runs
+void ora_build_tree_literal_f32(const float* p, int64_t n, int d, float* po, int32_t* s, int64_t* idx) {
+  build_tree<float>(p, n, d, po, s, idx);
+}
+void ora_build_tree_literal_f64(const double* p, int64_t n, int d, double* po, int32_t* s, int64_t* idx) {
   build_tree<double>(p, n, d, po, s, idx);
 }
 

@@ -82,8 +82,10 @@ def matern_kernel(*, p, variance, cutoff, r_min, r_max, n_bins, jitter=0.0):
 
 
 # ------------------------------------------------------------------- tree.py
-def build_tree(points):
-    """tree.py:153-184.  Returns (points, split_dims int32, indices int64)."""
+def build_tree(points, literal=False):
+    """tree.py:153-184.  Returns (points, split_dims int32, indices int64).  ``literal`` runs
+    the whole-array sort per level as the reference writes it (slow; for pinning the
+    default, which sorts each node's contiguous run of members independently)."""
     pts = np.ascontiguousarray(points)
     if pts.dtype not in (np.float32, np.float64):
         pts = pts.astype(np.float64)
@@ -91,7 +93,10 @@ def build_tree(points):
     po = np.empty_like(pts)
     sd = np.empty(n, np.int32)
     idx = np.empty(n, np.int64)
-    fn = lib().ora_build_tree_f32 if pts.dtype == np.float32 else lib().ora_build_tree_f64
+    if literal:
+        fn = lib().ora_build_tree_literal_f32 if pts.dtype == np.float32 else lib().ora_build_tree_literal_f64
+    else:
+        fn = lib().ora_build_tree_f32 if pts.dtype == np.float32 else lib().ora_build_tree_f64
     fn(_ptr(pts), _i64(n), ctypes.c_int(d), _ptr(po), _ptr(sd), _ptr(idx))
     return po, sd, idx
 

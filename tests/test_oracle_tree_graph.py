@@ -15,6 +15,27 @@ def test_build_tree_simple():
     assert np.array_equal(idx, [5, 2, 8, 1, 7, 4, 9, 0, 6, 3])
 
 
+@pytest.mark.parametrize("n,d,seed,kind", [(1, 3, 0, "c"), (2, 2, 1, "c"), (17, 3, 3, "c"), (1000, 6, 4, "c"),
+                                           (65537, 3, 5, "c"), (40000, 2, 6, "c"), (20000, 3, 7, "lattice"),
+                                           (5000, 2, 8, "ref")])
+def test_build_tree_runs_equals_literal(n, d, seed, kind):
+    # the default build sorts each node's contiguous run independently; the literal build
+    # sorts the whole array by (node, value, position) per level as tree.py:153-184 does
+    from helpers import clustered_points
+
+    if kind == "c":
+        p = clustered_points(n, d=d, seed=seed)
+    elif kind == "ref":
+        p = ref_points((n, d))
+    else:
+        rng = np.random.default_rng(seed)
+        p = (rng.integers(0, 5, (n, d)) * 0.5).astype(np.float64)
+        p[::7, 1] = -0.0
+    a, b = O.build_tree(p), O.build_tree(p, literal=True)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
 def test_rng_canary():
     # tests/test_tree.py:29 pins the JAX RNG restatement used below
     assert np.isclose(ref_points((100000, 3))[0, 0], -0.61777326, rtol=1e-8)
