@@ -19,6 +19,7 @@ from .refine import (
     refine_logdet,
     compute_cov_matrix,
 )
+from .store import save_graph, load_graph
 from . import extras
 
 __all__ = [
@@ -41,5 +42,7 @@ __all__ = [
     "refine_inv",
     "refine_logdet",
     "compute_cov_matrix",
+    "save_graph",
+    "load_graph",
     "extras",
 ]

@@ -445,6 +445,66 @@ void ora_query_neighbors_f64(const double* p, const int32_t* split, int64_t n, i
     query_one<double>(p, split, d, qidx[i], midx[i], k, out + i * k, visited ? visited + i : nullptr);
 }
 
+// Brute force for tests/test_tree.py:44-59's property at any size: for each query q (a
+// tree position), the k smallest squared distances among points [0, q) (the preceding
+// points), with the kNN's own FP32 arithmetic (sqdist, separately rounded), ordered by
+// (distance^2, index).  One pass over the points serves all queries: every thread keeps
+// a k-list per query for its share of the points, and the lists are merged at the end.
+void ora_brute_preceding_knn_f32(const float* p, int64_t n, int d, const int64_t* queries, int64_t nq, int k,
+                                 float* out_d2, int64_t* out_idx) {
+  using Item = std::pair<float, int64_t>;
+  auto less = [](const Item& a, const Item& b) { return a.first < b.first || (a.first == b.first && a.second < b.second); };
+  int nthreads = 1;
+#ifdef _OPENMP
+  nthreads = omp_get_max_threads();
+#endif
+  std::vector<std::vector<Item>> lists((size_t)nthreads * nq);
+  int64_t qmax = 0;
+  for (int64_t j = 0; j < nq; ++j) qmax = std::max(qmax, queries[j]);
+  const int64_t limit = std::min(n, qmax);
+#pragma omp parallel
+  {
+    int t = 0;
+#ifdef _OPENMP
+    t = omp_get_thread_num();
+#endif
+    std::vector<Item>* mine = &lists[(size_t)t * nq];
+    std::vector<float> worst(nq, std::numeric_limits<float>::infinity());
+    std::vector<int64_t> worst_idx(nq, std::numeric_limits<int64_t>::max());
+#pragma omp for schedule(static)
+    for (int64_t i = 0; i < limit; ++i) {
+      const float* x = p + i * d;
+      for (int64_t j = 0; j < nq; ++j) {
+        if (i >= queries[j]) continue;
+        const float s = sqdist(x, p + queries[j] * d, d);
+        if (s > worst[j] || (s == worst[j] && i > worst_idx[j])) continue;
+        std::vector<Item>& L = mine[j];
+        if ((int)L.size() < k) {
+          L.push_back({s, i});
+          std::push_heap(L.begin(), L.end(), less);
+        } else {
+          std::pop_heap(L.begin(), L.end(), less);
+          L.back() = {s, i};
+          std::push_heap(L.begin(), L.end(), less);
+        }
+        if ((int)L.size() == k) {
+          worst[j] = L.front().first;
+          worst_idx[j] = L.front().second;
+        }
+      }
+    }
+  }
+  for (int64_t j = 0; j < nq; ++j) {
+    std::vector<Item> all;
+    for (int t = 0; t < nthreads; ++t) all.insert(all.end(), lists[(size_t)t * nq + j].begin(), lists[(size_t)t * nq + j].end());
+    std::sort(all.begin(), all.end(), less);
+    for (int c = 0; c < k; ++c) {
+      out_d2[j * k + c] = c < (int)all.size() ? all[c].first : std::numeric_limits<float>::infinity();
+      out_idx[j * k + c] = c < (int)all.size() ? all[c].second : -1;
+    }
+  }
+}
+
 // ------------------------------------------------------------------ graph.py
 // graph.py:101-120 compute_depths.  The reference iterates depth[i] = 1 + max
 // depth[nbrs] to a fixed point; for a topologically ordered graph that fixed point

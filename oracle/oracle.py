@@ -124,6 +124,31 @@ def query_preceding_neighbors(points, split_dims, *, n0, k):
     return query_neighbors(points, split_dims, q, q, k=k)
 
 
+def brute_preceding_knn(points, queries, *, k):
+    """The k smallest FP32 squared distances (and their tree positions) among the points
+    preceding each query position, ordered by (distance^2, index): what the preceding kNN
+    (tree.py:41-63) must find, computed by brute force in one pass over the points."""
+    pts = np.ascontiguousarray(points, dtype=np.float32)
+    q = np.ascontiguousarray(queries, dtype=np.int64)
+    n, d = pts.shape
+    d2 = np.empty((len(q), k), np.float32)
+    idx = np.empty((len(q), k), np.int64)
+    lib().ora_brute_preceding_knn_f32(_ptr(pts), _i64(n), ctypes.c_int(d), _ptr(q), _i64(len(q)), ctypes.c_int(k),
+                                      _ptr(d2), _ptr(idx))
+    return d2, idx
+
+
+def sqdist_f32(points, a, b):
+    """FP32 squared distances between rows a and b of points, summed left to right with
+    separate roundings (the kNN's arithmetic)."""
+    pts = np.asarray(points, dtype=np.float32)
+    diff = pts[a] - pts[b]
+    s = diff[..., 0] * diff[..., 0]
+    for j in range(1, pts.shape[1]):
+        s = s + diff[..., j] * diff[..., j]
+    return s
+
+
 # ------------------------------------------------------------------ graph.py
 def compute_depths(neighbors, *, n0):
     """graph.py:101-120 (forward pass == the reference's fixed point for valid graphs)."""
@@ -306,8 +331,11 @@ def refine_logdet(points, neighbors, offsets, covariance):
 def local_conditional(points, neighbors_row, fine_index, covariance, neighbor_values):
     """Conditional mean and std of one refined point given its neighbours' values."""
     bins, vals = _cov(covariance)
-    p = _f64(points)
-    nb = np.ascontiguousarray(neighbors_row, dtype=np.int32)
+    rows = np.asarray(neighbors_row, dtype=np.int64)
+    # only the k + 1 points involved are converted (the point set may be large)
+    p = _f64(np.concatenate([np.asarray(points)[rows], np.asarray(points)[int(fine_index)][None]]))
+    nb = np.arange(len(rows), dtype=np.int32)
+    fine_index = len(rows)
     nv = _f64(neighbor_values)
     mean = ctypes.c_double()
     std = ctypes.c_double()

@@ -1,0 +1,57 @@
+"""The serving stream at its stated size: 2,000 generate / inverse requests with Poisson
+arrivals (N log-uniform in [1e3, 1e7], d in {2, 3}, uniform or 8-cluster points, k in
+{8, 16, 32}, n0 = max(k, min(1000, N // 100)), RBF or Matern-3/2 tables of 1000 bins with
+scale log-uniform in [0.01, 0.3], 70% reusing a resident graph; HBM budget 200 GB), served
+in real time by the continuous-batching server.  A random sample of 200 requests is run
+again alone, each on a freshly built graph: every output equals its output in the mix bit
+for bit."""
+import numpy as np
+import pytest
+import torch
+
+import graphgp_amd as gp
+from graphgp_amd.serving import Server, materialize, synthetic_mix
+
+pytestmark = pytest.mark.gpu
+
+N_REQUESTS = 2000
+RATE = 150.0  # requests/s: about 90% of this server's batch capacity on the mix
+
+
+def test_c4_stream_alone_equals_mix():
+    specs = synthetic_mix(N_REQUESTS, rate_per_s=RATE, seed=5)
+    reqs = materialize(specs)
+    torch.cuda.synchronize()
+    srv = Server(hbm_budget_bytes=200 * 10**9)
+    stats = srv.run(reqs, realtime=True)
+    srv.close()
+    del srv  # its resident graphs
+    torch.cuda.empty_cache()
+    print(f"[C4] served {stats['requests']} requests in {stats['wall_s']:.1f} s: "
+          f"{stats['points_per_s'] / 1e6:.1f} M points/s, p50 {stats['p50_ms']:.1f} ms, "
+          f"p99 N<=1e6 {stats['p99_ms_small']:.1f} ms", flush=True)
+    assert stats["requests"] == N_REQUESTS
+    rng = np.random.default_rng(55)
+    sample = sorted(rng.choice(N_REQUESTS, 200, replace=False))
+    graphs = {}
+    nan_requests = []
+    for i in sample:
+        r = reqs[i]
+        if r.graph_key not in graphs:
+            graphs[r.graph_key] = gp.build_graph(r.points, n0=r.n0, k=r.k, schedule=False)
+        g = graphs[r.graph_key]
+        fn = gp.generate if r.op == "generate" else gp.generate_inv
+        alone = fn(g, r.covariance, r.data, precision=r.precision)
+        assert alone.dtype == r.result.dtype == torch.float32
+        assert torch.equal(alone.view(torch.int32), r.result.view(torch.int32)), (r.rid, r.op, r.n)
+        if torch.isnan(r.result).any():
+            # the reference's rule (refine.py:55, :182): a local covariance that is not positive
+            # definite makes every value NaN.  It must be a property of the request's table and
+            # points, not of the FP32 arithmetic: the FP64 path fails too.
+            assert torch.isnan(r.result).all(), (r.rid, r.op, r.n)
+            ref64 = fn(g, r.covariance, r.data.double(), precision="fp64")
+            assert torch.isnan(ref64).all(), (r.rid, r.op, r.n)
+            nan_requests.append(r.rid)
+    print(f"[C4] {len(sample)} sampled requests ({len(graphs)} graphs) equal alone and in the mix; "
+          f"all-NaN (not positive definite in FP64 too): {nan_requests}", flush=True)
+    assert len(nan_requests) <= len(sample) // 20
