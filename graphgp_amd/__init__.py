@@ -11,6 +11,8 @@ from .refine import (
     generate_inv,
     generate_logdet,
     generate_vjp,
+    generate_inv_vjp,
+    generate_logdet_vjp,
     generate_dense,
     generate_dense_inv,
     generate_dense_logdet,
@@ -35,6 +37,8 @@ __all__ = [
     "generate_inv",
     "generate_logdet",
     "generate_vjp",
+    "generate_inv_vjp",
+    "generate_logdet_vjp",
     "generate_dense",
     "generate_dense_inv",
     "generate_dense_logdet",

@@ -51,6 +51,10 @@ _SIGS = {
     "ggp_generate_vjp_workspace": (_c_size, [_i64, _int, _i64, _int, _int, _int]),
     "ggp_generate_vjp": (_int, [_p, _int, _i64, _int, _p, _i64, _int, _p, _p, _p, _int, _p, _int, _p, _int, _p,
                                 _int, _p, _int, _p, _int, _p, _c_size, _p]),
+    "ggp_generate_inv_vjp": (_int, [_p, _int, _i64, _int, _p, _i64, _int, _p, _p, _p, _int, _p, _int, _p, _int, _p,
+                                    _int, _p, _int, _p, _c_size, _p]),
+    "ggp_generate_logdet_vjp": (_int, [_p, _int, _i64, _int, _p, _i64, _int, _p, _p, _int, _p, _p, _int, _p, _c_size,
+                                       _p]),
     "ggp_generate_inv": (_int, [_p, _int, _i64, _int, _p, _i64, _int, _p, _p, _p, _int, _p, _int, _p, _int, _int,
                                 _p, _c_size, _p]),
     "ggp_generate_logdet": (_int, [_p, _int, _i64, _int, _p, _i64, _int, _p, _p, _int, _p, _int, _p, _c_size, _p]),

@@ -349,7 +349,41 @@ int ggp_generate_vjp(const void* points, int pdtype, int64_t n, int d, const int
   io.gxi = xi_bar;
   io.gxi_f64 = xi_bar_dtype == GGP_F64;
   io.gtab = cov_vals_bar;
-  return vjp_run(p, t64_of(precision), io, workspace, workspace_bytes, S(stream));
+  return vjp_run(p, 0, t64_of(precision), io, workspace, workspace_bytes, S(stream));
+}
+
+int ggp_generate_inv_vjp(const void* points, int pdtype, int64_t n, int d, const int32_t* neighbors, int64_t n0,
+                         int k, const int32_t* indices, const double* cov_bins, const double* cov_vals, int n_bins,
+                         const void* values, int values_dtype, const void* xi_bar, int xi_bar_dtype,
+                         void* values_bar, int values_bar_dtype, double* cov_vals_bar, int precision,
+                         void* workspace, size_t workspace_bytes, void* stream) {
+  GGP_TRY(check_refine_args(n, n0, k, precision, pdtype));
+  GGP_TRY(check_dtype(values_dtype, "generate_inv_vjp values"));
+  GGP_TRY(check_dtype(xi_bar_dtype, "generate_inv_vjp xi_bar"));
+  GGP_TRY(check_dtype(values_bar_dtype, "generate_inv_vjp values_bar"));
+  RefineProblem p = problem(points, pdtype, n, d, neighbors, n0, k, indices, cov_bins, cov_vals, n_bins, precision, 1);
+  VjpIO io{};
+  io.values = values;
+  io.values_f64 = values_dtype == GGP_F64;
+  io.gvalues = xi_bar;
+  io.g_f64 = xi_bar_dtype == GGP_F64;
+  io.gxi = values_bar;
+  io.gxi_f64 = values_bar_dtype == GGP_F64;
+  io.gtab = cov_vals_bar;
+  return vjp_run(p, 1, t64_of(precision), io, workspace, workspace_bytes, S(stream));
+}
+
+int ggp_generate_logdet_vjp(const void* points, int pdtype, int64_t n, int d, const int32_t* neighbors, int64_t n0,
+                            int k, const double* cov_bins, const double* cov_vals, int n_bins,
+                            const double* logdet_bar, double* cov_vals_bar, int precision, void* workspace,
+                            size_t workspace_bytes, void* stream) {
+  GGP_TRY(check_refine_args(n, n0, k, precision, pdtype));
+  GGP_CHECK_ARG(logdet_bar != nullptr, "generate_logdet_vjp: logdet_bar is required");
+  RefineProblem p = problem(points, pdtype, n, d, neighbors, n0, k, nullptr, cov_bins, cov_vals, n_bins, precision, 1);
+  VjpIO io{};
+  io.gscalar = logdet_bar;
+  io.gtab = cov_vals_bar;
+  return vjp_run(p, 2, t64_of(precision), io, workspace, workspace_bytes, S(stream));
 }
 
 int ggp_generate_inv(const void* points, int pdtype, int64_t n, int d, const int32_t* neighbors, int64_t n0, int k,

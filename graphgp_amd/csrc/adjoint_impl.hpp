@@ -38,12 +38,15 @@ struct AdjointArgs {
   const int32_t* perm;
   const void* xi;  // caller order
   int xi_f64;
-  const T* vals;       // forward values, graph order
+  const T* vals;       // generate: forward values; inverse: its input values (graph order)
   T* vbar;             // adjoint of the values, graph order (accumulated)
   uint32_t* children;  // unfinished children per point
   void* gxi;           // caller order
   int gxi_f64;
   double* gtab;  // (n_bins) table gradient, FP64
+  const void* cot;       // inverse: cotangent of xi (caller order)
+  int cot_f64;
+  const double* ld_bar;  // log-det: cotangent of the log-det (device scalar)
   const TabE<T>* tab;
   const TabMeta* meta;
   uint32_t* ticket;
@@ -52,7 +55,7 @@ struct AdjointArgs {
   int nbins;
 };
 
-template <typename T, int KK, int P, int D, bool FAST>
+template <typename T, int KK, int P, int D, bool FAST, int MODE>
 __global__ __launch_bounds__(256, 2) void adjoint_kernel(AdjointArgs<T> a) {
   static_assert(KK % P == 0, "KK must be a multiple of P");
   constexpr int R = KK / P;
@@ -228,10 +231,10 @@ __global__ __launch_bounds__(256, 2) void adjoint_kernel(AdjointArgs<T> a) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       w[r] = bv[r];
-      z[r] = (valid && P * r + lane < k) ? a.vals[nid[r]] : T(0);
+      z[r] = (MODE != kLogdet && valid && P * r + lane < k) ? a.vals[nid[r]] : T(0);
     }
     solve(w);
-    solve(z);
+    if constexpr (MODE != kLogdet) solve(z);
     // sigma^2 = c - b . w  (group sum over the P lanes)
     double bw = 0.0;
 #pragma unroll
@@ -240,85 +243,124 @@ __global__ __launch_bounds__(256, 2) void adjoint_kernel(AdjointArgs<T> a) {
     bw = group_sum<P>(bw);
     const double c0 = (double)lookup2(T(0));
     const double sigma = sqrt(c0 - bw);
-    const double xi = valid ? load_as<double>(a.xi, a.xi_f64, a.perm ? (int64_t)a.perm[fine] : fine) : 0.0;
-
-    // wait until every child has added its share to vbar[fine]; the polling loop is
-    // wavefront-uniform so a point whose child sits in the same wavefront never blocks it
     const int gbase = (threadIdx.x & 63) & ~(P - 1);
     const uint64_t gmask = (P == 64) ? ~0ull : ((1ull << P) - 1ull);
-    bool done = !valid;
-    for (uint32_t it = 0;; ++it) {
-      if (!done) {
-        const uint32_t left = __hip_atomic_load(&a.children[fine], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint64_t bal = __ballot(left == 0);
-        if (((bal >> gbase) & gmask) == gmask) {
-          const double s = (double)__hip_atomic_load(&a.vbar[fine], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          if (lane == 0) {
-            const double gx = sigma * s;
-            const int64_t dst = a.perm ? (int64_t)a.perm[fine] : fine;
-            if (a.gxi_f64) ((double*)a.gxi)[dst] = gx;
-            else ((float*)a.gxi)[dst] = (float)gx;
-          }
-#pragma unroll
-          for (int r = 0; r < R; ++r)
-            if (P * r + lane < k)
-              __hip_atomic_fetch_add(&a.vbar[nid[r]], (T)(s * (double)w[r]), __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_AGENT);
-          // every addition has completed at its device-scope destination before any count drops
-          __atomic_signal_fence(__ATOMIC_SEQ_CST);
-          __builtin_amdgcn_s_waitcnt(0);
-          __atomic_signal_fence(__ATOMIC_SEQ_CST);
-#pragma unroll
-          for (int r = 0; r < R; ++r)
-            if (P * r + lane < k)
-              __hip_atomic_fetch_sub(&a.children[nid[r]], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          if (s != 0.0) {
-            const double sx = s * xi;
-            T c[R][D], cf[D];
-            load_rows(c, cf);
-            // entries of A (diagonal and below), then b, then c.  Unrolled over the row
-            // slot jr only; the column's lane js is a run-time loop (ds_bpermute broadcast),
-            // which keeps the register footprint of this phase small.
-            static_for<0, R>([&](auto jrc) {
-              constexpr int jr = decltype(jrc)::value;
+
+    // table gradient of one point from the cotangents of its weights and variance: with
+    // ybar = A^{-1} wbar = al * z and q = beta the cotangent of sigma^2,
+    //   dA_aa = -al w_a z_a + be w_a^2,  dA_ab = -al (w_a z_b + w_b z_a) + 2 be w_a w_b,
+    //   db_a = al z_a - 2 be w_a,  dc = be.
+    // Called by all lanes of the point's group together (cross-lane reads stay inside it).
+    auto scatter = [&](double al, double be) {
+      T c[R][D], cf[D];
+      load_rows(c, cf);
+      // entries of A (diagonal and below), then b, then c.  Unrolled over the row slot jr
+      // only; the column's lane js is a run-time loop (ds_bpermute broadcast), which keeps
+      // the register footprint of this phase small.
+      static_for<0, R>([&](auto jrc) {
+        constexpr int jr = decltype(jrc)::value;
 #pragma unroll 1
-              for (int js = 0; js < P; ++js) {
-                const int j = P * jr + js;
-                if (j < k) {
-                  const int src = gbase + js;
-                  T cj[D];
+        for (int js = 0; js < P; ++js) {
+          const int j = P * jr + js;
+          if (j < k) {
+            const int src = gbase + js;
+            T cj[D];
 #pragma unroll
-                  for (int t = 0; t < D; ++t) cj[t] = __shfl(c[jr][t], src, 64);
-                  const double wj = (double)__shfl(w[jr], src, 64);
-                  const double zj = (double)__shfl(z[jr], src, 64);
+            for (int t = 0; t < D; ++t) cj[t] = __shfl(c[jr][t], src, 64);
+            const double wj = (double)__shfl(w[jr], src, 64);
+            const double zj = (double)__shfl(z[jr], src, 64);
 #pragma unroll
-                  for (int r = jr; r < R; ++r) {
-                    const int i = P * r + lane;
-                    if (i >= j && i < k) {
-                      const double wi = (double)w[r], zi = (double)z[r];
-                      const double g = (i == j) ? (-s * wi * zi + sx * wi * wi / (2.0 * sigma))
-                                                : (-s * (wi * zj + wj * zi) + sx * wi * wj / sigma);
-                      grad_entry(pair_dist2<T, D>(c[r], cj), g);
-                    }
-                  }
-                }
+            for (int r = jr; r < R; ++r) {
+              const int i = P * r + lane;
+              if (i >= j && i < k) {
+                const double wi = (double)w[r], zi = (double)z[r];
+                const double g = (i == j) ? (-al * wi * zi + be * wi * wi)
+                                          : (-al * (wi * zj + wj * zi) + 2.0 * be * wi * wj);
+                grad_entry(pair_dist2<T, D>(c[r], cj), g);
               }
-            });
+            }
+          }
+        }
+      });
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (P * r + lane < k) grad_entry(pair_dist2<T, D>(c[r], cf), al * (double)z[r] - 2.0 * be * (double)w[r]);
+      if (lane == 0) atomicAdd(&hist[0], be);
+    };
+
+    if constexpr (MODE == kGenerate) {
+      const double xi = valid ? load_as<double>(a.xi, a.xi_f64, a.perm ? (int64_t)a.perm[fine] : fine) : 0.0;
+      // wait until every child has added its share to vbar[fine]; the polling loop is
+      // wavefront-uniform so a point whose child sits in the same wavefront never blocks it
+      bool done = !valid;
+      for (uint32_t it = 0;; ++it) {
+        if (!done) {
+          const uint32_t left = __hip_atomic_load(&a.children[fine], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          const uint64_t bal = __ballot(left == 0);
+          if (((bal >> gbase) & gmask) == gmask) {
+            const double s = (double)__hip_atomic_load(&a.vbar[fine], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0) {
+              const double gx = sigma * s;
+              const int64_t dst = a.perm ? (int64_t)a.perm[fine] : fine;
+              if (a.gxi_f64) ((double*)a.gxi)[dst] = gx;
+              else ((float*)a.gxi)[dst] = (float)gx;
+            }
 #pragma unroll
             for (int r = 0; r < R; ++r)
               if (P * r + lane < k)
-                grad_entry(pair_dist2<T, D>(c[r], cf), s * (double)z[r] - sx * (double)w[r] / sigma);
-            if (lane == 0) atomicAdd(&hist[0], sx / (2.0 * sigma));
+                __hip_atomic_fetch_add(&a.vbar[nid[r]], (T)(s * (double)w[r]), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            // every addition has completed at its device-scope destination before any count drops
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            __builtin_amdgcn_s_waitcnt(0);
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+              if (P * r + lane < k)
+                __hip_atomic_fetch_sub(&a.children[nid[r]], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // generate: v = w . u + sigma xi, so wbar = s u (ybar = s z) and the cotangent of
+            // sigma^2 is s xi / (2 sigma)
+            if (s != 0.0) scatter(s, s * xi / (2.0 * sigma));
+            done = true;
           }
-          done = true;
         }
+        if (__all(done)) break;
+        if (it > (1u << 22)) {
+          // a malformed graph: the gradient is undefined; the host turns it all into NaN
+          if (!done) atomicOr(&a.flags[1], 1u);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(8);
       }
-      if (__all(done)) break;
-      if (it > (1u << 22)) {
-        if (!done) atomicOr(&a.flags[1], 1u);
-        break;
+    } else if constexpr (MODE == kInverse) {
+      // xi_i = (v_i - w . u) / sigma with w . u = b . z; cotangent t of xi_i:
+      //   vbar_i += t / sigma, vbar[nbrs] += -t w / sigma, ybar = -(t / sigma) z,
+      //   cotangent of sigma^2 = -t xi_i / (2 sigma^2)
+      double bz = 0.0;
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (P * r + lane < k) bz += (double)bv[r] * (double)z[r];
+      bz = group_sum<P>(bz);
+      if (valid) {
+        const double vi = (double)a.vals[fine];
+        const double xo = (vi - bz) / sigma;
+        const double t = load_as<double>(a.cot, a.cot_f64, a.perm ? (int64_t)a.perm[fine] : fine);
+        const double al = -t / sigma;
+        if (lane == 0)
+          __hip_atomic_fetch_add(&a.vbar[fine], (T)(t / sigma), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          if (P * r + lane < k)
+            __hip_atomic_fetch_add(&a.vbar[nid[r]], (T)(al * (double)w[r]), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        if (t != 0.0) scatter(al, -t * xo / (2.0 * sigma * sigma));
       }
-      __builtin_amdgcn_s_sleep(8);
+    } else {
+      // log-det: sum log sigma_i with cotangent l: sigma^2 gets l / (2 sigma^2), w none
+      if (valid) {
+        const double l = *a.ld_bar;
+        if (l != 0.0) scatter(0.0, l / (2.0 * sigma * sigma));
+      }
     }
   }
   __syncthreads();
@@ -326,22 +368,20 @@ __global__ __launch_bounds__(256, 2) void adjoint_kernel(AdjointArgs<T> a) {
     if (hist[j] != 0.0) atomicAdd(&a.gtab[j], hist[j]);
 }
 
-template <typename T>
-int launch_adjoint(AdjointArgs<T> a, int kk, hipStream_t s, size_t lds, int grid);
-template <>
-int launch_adjoint<float>(AdjointArgs<float> a, int kk, hipStream_t s, size_t lds, int grid);
-template <>
-int launch_adjoint<double>(AdjointArgs<double> a, int kk, hipStream_t s, size_t lds, int grid);
+// Per (type, mode): the register configurations (d <= 3, k + 1 <= 32); kUnsupported
+// otherwise.  Instantiated in adjoint_k_*.hip, one file per type and mode.
+template <typename T, int MODE>
+int launch_adjoint_mode(AdjointArgs<T> a, int kk, hipStream_t s, size_t lds, int grid);
 
 // ------------------------------------------------------------------ launch
-template <typename T, int KK, int P, int D>
+template <typename T, int KK, int P, int D, int MODE>
 int launch_adjoint_config(AdjointArgs<T> a, hipStream_t s, size_t lds, int grid) {
   constexpr int64_t gpt = 256 / P;
   a.ntiles = (a.n_ref + gpt - 1) / gpt;
   if (a.ntiles == 0) return kOk;
   if ((int64_t)grid > a.ntiles) grid = (int)a.ntiles;
-  auto k1 = adjoint_kernel<T, KK, P, D, true>;
-  auto k0 = adjoint_kernel<T, KK, P, D, false>;
+  auto k1 = adjoint_kernel<T, KK, P, D, true, MODE>;
+  auto k0 = adjoint_kernel<T, KK, P, D, false, MODE>;
   if (lds > 65536) {
     GGP_HIP(hipFuncSetAttribute((const void*)k1, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     GGP_HIP(hipFuncSetAttribute((const void*)k0, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -352,5 +392,17 @@ int launch_adjoint_config(AdjointArgs<T> a, hipStream_t s, size_t lds, int grid)
   return kOk;
 }
 
+// at most four rows per lane: the adjoint keeps w, z and the factor live together
+#define GGP_ADJOINT_INSTANTIATE(T, MODE)                                                          \
+  template <>                                                                                     \
+  int launch_adjoint_mode<T, MODE>(AdjointArgs<T> a, int kk, hipStream_t s, size_t lds, int grid) { \
+    if (a.d > 3) return kUnsupported;                                                             \
+    if (kk <= 8) return launch_adjoint_config<T, 8, 2, 3, MODE>(a, s, lds, grid);                 \
+    if (kk <= 12) return launch_adjoint_config<T, 12, 4, 3, MODE>(a, s, lds, grid);               \
+    if (kk <= 16) return launch_adjoint_config<T, 16, 4, 3, MODE>(a, s, lds, grid);               \
+    if (kk <= 24) return launch_adjoint_config<T, 24, 8, 3, MODE>(a, s, lds, grid);               \
+    if (kk <= 32) return launch_adjoint_config<T, 32, 16, 3, MODE>(a, s, lds, grid);              \
+    return kUnsupported;                                                                          \
+  }
 
 }  // namespace ggp

@@ -34,6 +34,7 @@
 #include "refine_impl.hpp"
 #include "dense.hpp"
 #include "refine_ops.hpp"
+#include "generic_slab.hpp"
 
 namespace ggp {
 
@@ -129,8 +130,11 @@ namespace {
 template <typename T>
 __global__ __launch_bounds__(64) void refine_generic_kernel(RefineArgs<T> a, T* __restrict__ scratch) {
   const int k = a.k, m = k + 1;
-  T* K = scratch + (size_t)(blockIdx.x * 64 + threadIdx.x) * ((size_t)m * m + m);
-  T* u = K + (size_t)m * m;
+  // the block's systems interleaved by lane (generic_slab.hpp): K (m x m) then u (m)
+  const LaneSlab<T> S{scratch + (size_t)blockIdx.x * 64 * ((size_t)m * m + m) + threadIdx.x};
+  const size_t uo = (size_t)m * m;
+  auto K = [&](size_t e) -> T& { return S[e]; };
+  auto u = [&](int i) -> T& { return S[uo + i]; };
   const TabMeta meta = *a.meta;
   const T x1 = (T)meta.x1, x_last = (T)meta.x_last, f_last = (T)meta.f_last;
   __shared__ uint32_t s_tile;
@@ -157,7 +161,7 @@ __global__ __launch_bounds__(64) void refine_generic_kernel(RefineArgs<T> a, T* 
     };
     const T* gedges = reinterpret_cast<const T*>(a.tab + meta.nseg);
     auto look = [&](T r) { return table_lookup<T>(a.tab, gedges, meta.nseg, x_last, f_last, r); };
-    T skk = T(1);
+    T sigma = T(1);
     if (valid) {
       for (int i = 0; i < m; ++i) {
         T gi = incr ? look(dist(i, 0)) : T(0);
@@ -168,29 +172,20 @@ __global__ __launch_bounds__(64) void refine_generic_kernel(RefineArgs<T> a, T* 
             if (j == 0) v = (i == 0) ? (T)meta.f_zero : -gi;
             else v = (gi + gj) - v;
           }
-          K[(size_t)i * m + j] = v;
+          K((size_t)i * m + j) = v;
         }
       }
-      for (int c = 0; c < k; ++c) {
-        T dc = sqrt(K[(size_t)c * m + c]);
-        K[(size_t)c * m + c] = dc;
-        for (int i = c + 1; i < m; ++i) K[(size_t)i * m + c] /= dc;
-        for (int i = c + 1; i < m; ++i) {
-          T lic = K[(size_t)i * m + c];
-          for (int j = c + 1; j <= i; ++j) K[(size_t)i * m + j] -= lic * K[(size_t)j * m + c];
-        }
-      }
-      skk = K[(size_t)k * m + k];
+      // Cholesky of the whole joint system: its last pivot is sigma
+      slab_cholesky(S, m, m);
+      sigma = K((size_t)k * m + k);
     }
+    // u[k] - L[k, :k] . L_A^{-1} u[:k]
     auto forward = [&]() -> T {
-      for (int c = 0; c < k; ++c) {
-        T y = u[c] / K[(size_t)c * m + c];
-        for (int i = c + 1; i < m; ++i) u[i] -= K[(size_t)i * m + c] * y;
-      }
-      return u[k];
+      slab_solve_lower(S, k, m, uo);
+      return u(k) - slab_dot(S, (size_t)k * m, uo, k);
     };
     if (a.mode == kLogdet) {
-      double ld = valid ? 0.5 * log((double)skk) : 0.0;
+      double ld = valid ? log((double)sigma) : 0.0;
       if (valid && ld != ld) atomicOr(&a.flags[0], 1u);
       for (int o = 32; o > 0; o >>= 1) ld += __shfl_xor(ld, o, 64);
       if (threadIdx.x == 0) a.partials[tile] = ld;
@@ -199,11 +194,11 @@ __global__ __launch_bounds__(64) void refine_generic_kernel(RefineArgs<T> a, T* 
     const int64_t dst = valid ? (a.perm ? (int64_t)a.perm[fine] : fine) : 0;
     if (a.mode == kInverse) {
       if (valid) {
-        for (int i = 0; i < k; ++i) u[i] = a.vals[pid(i)];
-        u[k] = a.vals[fine];
+        for (int i = 0; i < k; ++i) u(i) = a.vals[pid(i)];
+        u(k) = a.vals[fine];
         if (incr)
-          for (int i = 1; i <= k; ++i) u[i] -= u[0];
-        T xi = forward() / sqrt(skk);
+          for (int i = 1; i <= k; ++i) u(i) -= u(0);
+        T xi = forward() / sigma;
         if (xi != xi) atomicOr(&a.flags[0], 1u);
         if (a.out_f64) ((double*)a.out)[dst - a.io_base] = (double)xi;
         else ((float*)a.out)[dst - a.io_base] = (float)xi;
@@ -218,16 +213,16 @@ __global__ __launch_bounds__(64) void refine_generic_kernel(RefineArgs<T> a, T* 
         for (int i = 0; i < k; ++i) {
           T v = __hip_atomic_load(&a.vals[pid(i)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           ok = ok && !Sentinel<T>::is(v);
-          u[i] = v;
+          u(i) = v;
         }
         if (ok) {
-          u[k] = T(0);
+          u(k) = T(0);
           T v0 = T(0);
           if (incr) {
-            v0 = u[0];
-            for (int i = 1; i < k; ++i) u[i] -= v0;
+            v0 = u(0);
+            for (int i = 1; i < k; ++i) u(i) -= v0;
           }
-          T v = v0 - forward() + sqrt(skk) * xi;
+          T v = v0 - forward() + sigma * xi;
           if (v != v) {
             atomicOr(&a.flags[0], 1u);
             v = __builtin_nan("");

@@ -36,19 +36,21 @@ struct RefineIO {
 };
 
 struct VjpIO {
-  const void* xi;  // xi of the forward pass (caller order, N)
+  const void* xi;  // generate: xi of the forward pass (caller order, N)
   int xi_f64;
-  const void* values;  // values of the forward pass (caller order, N)
+  const void* values;  // generate: values of the forward pass; inverse: its input values (caller order, N)
   int values_f64;
-  const void* gvalues;  // cotangent of the values (caller order, N)
+  const void* gvalues;  // generate: cotangent of the values; inverse: cotangent of xi (caller order, N)
   int g_f64;
-  void* gxi;  // out: cotangent of xi (caller order, N)
+  const double* gscalar;  // log-det: cotangent of the log-det (device scalar)
+  void* gxi;  // out -- generate: cotangent of xi; inverse: cotangent of the values (caller order, N)
   int gxi_f64;
   double* gtab;  // out: cotangent of the covariance table values (n_bins)
 };
 
 size_t vjp_workspace(const RefineProblem& p, int t64);
-int vjp_run(const RefineProblem& p, int t64, const VjpIO& io, void* ws, size_t ws_bytes, hipStream_t s);
+// mode: kGenerate (0), kInverse (1) or kLogdet (2) of refine_impl.hpp's RefineMode
+int vjp_run(const RefineProblem& p, int mode, int t64, const VjpIO& io, void* ws, size_t ws_bytes, hipStream_t s);
 
 int refine_points_per_tile(int t64, int kk, int d);
 size_t refine_workspace(const RefineProblem& p, int mode, int t64);

@@ -133,9 +133,74 @@ def _generate_vjp_raw(pts, nb, perm, bins, vals, x, v, g, prec):
     return gxi, gvals
 
 
+def generate_inv_vjp(graph: Graph, covariance, values, xi_bar, *, precision=None):
+    """Reverse mode of ``generate_inv``: what JAX autodiff of the reference's ``generate_inv``
+    (refine.py:162-183, 186-195, 198-231) returns for the cotangent ``xi_bar``.
+
+    ``values`` is the inverse's input (caller order).  Returns ``(values_bar, cov_vals_bar)``:
+    values_bar (N,) in the compute dtype, cov_vals_bar (n_bins,) FP64.
+    """
+    pts = as_points(graph.points)
+    v = as_device(values)
+    t = as_device(xi_bar)
+    n, d = pts.shape
+    if v.numel() != n or t.numel() != n:
+        raise ValueError("Length of values and xi_bar must match number of points in graph.")
+    nb = as_index(graph.neighbors)
+    bins, vals = as_covariance(covariance)
+    prec = _precision(precision, v, pts)
+    perm = as_index(graph.indices) if graph.indices is not None else None
+    return _generate_inv_vjp_raw(pts, nb, perm, bins, vals, v, t, prec)
+
+
+def _generate_inv_vjp_raw(pts, nb, perm, bins, vals, v, t, prec):
+    n, d = pts.shape
+    n_ref, k = nb.shape
+    n0 = n - n_ref
+    gv = torch.empty(n, dtype=_out_dtype(prec), device=pts.device)
+    gtab = torch.empty(bins.numel(), dtype=torch.float64, device=pts.device)
+    L = _lib.lib()
+    ws = _lib.workspace(L.ggp_generate_vjp_workspace(n, d, n0, k, bins.numel(), _lib.PREC[prec]))
+    _lib.check(L.ggp_generate_inv_vjp(_lib.ptr(pts), _lib.dtype_code(pts), n, d, _lib.ptr(nb), n0, k, _lib.ptr(perm),
+                                      _lib.ptr(bins), _lib.ptr(vals), bins.numel(), _lib.ptr(v), _lib.dtype_code(v),
+                                      _lib.ptr(t), _lib.dtype_code(t), _lib.ptr(gv), _lib.dtype_code(gv),
+                                      _lib.ptr(gtab), _lib.PREC[prec], _lib.ptr(ws), ws.numel(), _lib.stream()),
+               "generate_inv_vjp")
+    return gv, gtab
+
+
+def generate_logdet_vjp(graph: Graph, covariance, logdet_bar=1.0, *, precision=None):
+    """Reverse mode of ``generate_logdet``: the cotangent of the covariance table values
+    (n_bins,) FP64 for the log-det's cotangent ``logdet_bar`` (refine.py:234-275)."""
+    pts = as_points(graph.points)
+    nb = as_index(graph.neighbors)
+    bins, vals = as_covariance(covariance)
+    prec = _precision(precision, pts)
+    lb = torch.as_tensor(logdet_bar, dtype=torch.float64, device=pts.device).reshape(())
+    return _generate_logdet_vjp_raw(pts, nb, bins, vals, lb, prec)
+
+
+def _generate_logdet_vjp_raw(pts, nb, bins, vals, lb, prec):
+    n, d = pts.shape
+    n_ref, k = nb.shape
+    n0 = n - n_ref
+    gtab = torch.empty(bins.numel(), dtype=torch.float64, device=pts.device)
+    lb = lb.to(torch.float64).contiguous()
+    L = _lib.lib()
+    ws = _lib.workspace(L.ggp_generate_vjp_workspace(n, d, n0, k, bins.numel(), _lib.PREC[prec]))
+    _lib.check(L.ggp_generate_logdet_vjp(_lib.ptr(pts), _lib.dtype_code(pts), n, d, _lib.ptr(nb), n0, k,
+                                         _lib.ptr(bins), _lib.ptr(vals), bins.numel(), _lib.ptr(lb), _lib.ptr(gtab),
+                                         _lib.PREC[prec], _lib.ptr(ws), ws.numel(), _lib.stream()),
+               "generate_logdet_vjp")
+    return gtab
+
+
 def generate_inv(graph: Graph, covariance, values, *, cuda: bool = False, precision=None,
                  return_counters: bool = False):
-    """Inverse of ``generate`` (refine.py:162-183): values (N,) -> xi (N,), caller order."""
+    """Inverse of ``generate`` (refine.py:162-183): values (N,) -> xi (N,), caller order.
+
+    Differentiable with respect to ``values`` and the covariance values when either
+    requires grad (backward: ``generate_inv_vjp``)."""
     pts = as_points(graph.points)
     v = as_device(values)
     n, d = pts.shape
@@ -147,6 +212,19 @@ def generate_inv(graph: Graph, covariance, values, *, cuda: bool = False, precis
     bins, vals = as_covariance(covariance)
     prec = _precision(precision, v, pts)
     perm = as_index(graph.indices) if graph.indices is not None else None
+    if torch.is_grad_enabled() and (v.requires_grad or vals.requires_grad):
+        if return_counters:
+            raise ValueError("return_counters is not available on the differentiable path")
+        from .autograd import GenerateInvFunction
+
+        return GenerateInvFunction.apply(v, vals, pts, nb, perm, bins, prec)
+    return _generate_inv_raw(pts, nb, perm, bins, vals, v, prec, return_counters)
+
+
+def _generate_inv_raw(pts, nb, perm, bins, vals, v, prec, return_counters=False):
+    n, d = pts.shape
+    n_ref, k = nb.shape
+    n0 = n - n_ref
     out = torch.empty(n, dtype=_out_dtype(prec), device=pts.device)
     L = _lib.lib()
     ws = _lib.workspace(L.ggp_refine_workspace(n, d, n0, k, bins.numel(), _lib.PREC[prec], 1))
@@ -158,14 +236,25 @@ def generate_inv(graph: Graph, covariance, values, *, cuda: bool = False, precis
 
 
 def generate_logdet(graph: Graph, covariance, *, cuda: bool = False, precision=None):
-    """Log-determinant of ``generate`` (refine.py:234-240), summed in FP64; a 0-d tensor."""
+    """Log-determinant of ``generate`` (refine.py:234-240), summed in FP64; a 0-d tensor.
+
+    Differentiable with respect to the covariance values when they require grad
+    (backward: ``generate_logdet_vjp``)."""
     pts = as_points(graph.points)
-    n, d = pts.shape
     nb = as_index(graph.neighbors)
-    n_ref, k = nb.shape
-    n0 = n - n_ref
     bins, vals = as_covariance(covariance)
     prec = _precision(precision, pts)
+    if torch.is_grad_enabled() and vals.requires_grad:
+        from .autograd import GenerateLogdetFunction
+
+        return GenerateLogdetFunction.apply(vals, pts, nb, bins, prec)
+    return _generate_logdet_raw(pts, nb, bins, vals, prec)
+
+
+def _generate_logdet_raw(pts, nb, bins, vals, prec):
+    n, d = pts.shape
+    n_ref, k = nb.shape
+    n0 = n - n_ref
     out = torch.empty((), dtype=torch.float64, device=pts.device)
     L = _lib.lib()
     ws = _lib.workspace(L.ggp_refine_workspace(n, d, n0, k, bins.numel(), _lib.PREC[prec], 1))

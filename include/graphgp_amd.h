@@ -150,13 +150,35 @@ int ggp_generate_logdet(const void* points, int pdtype, int64_t n, int d, const 
  * (refine.py:21-56, 79-159) returns for the cotangent values_bar (n, caller order):
  * xi_bar (n, caller order, dtype xi_bar_dtype) and cov_vals_bar (n_bins, FP64, the cotangent
  * of cov_vals).  xi and values are the forward pass's input and output.  precision
- * GGP_PREC_FP64 works in FP64; the FP32 modes in FP32 (FP64 seed block). */
+ * GGP_PREC_FP64 works in FP64; the FP32 modes in FP32 (FP64 seed block).  Any k and d (a
+ * register kernel for d <= 3, k + 1 <= 32, else one lane per point); n0 <= 20480.  A
+ * malformed graph (a wait that gives up) makes xi_bar and cov_vals_bar all NaN. */
 size_t ggp_generate_vjp_workspace(int64_t n, int d, int64_t n0, int k, int n_bins, int precision);
 int ggp_generate_vjp(const void* points, int pdtype, int64_t n, int d, const int32_t* neighbors, int64_t n0, int k,
                      const int32_t* indices, const double* cov_bins, const double* cov_vals, int n_bins,
                      const void* xi, int xi_dtype, const void* values, int values_dtype, const void* values_bar,
                      int values_bar_dtype, void* xi_bar, int xi_bar_dtype, double* cov_vals_bar, int precision,
                      void* workspace, size_t workspace_bytes, void* stream);
+
+/* generate_inv_vjp: reverse mode of generate_inv -- what JAX autodiff of graphgp.generate_inv
+ * (refine.py:162-183, 186-195, 198-231) returns for the cotangent xi_bar (n, caller order):
+ * values_bar (n, caller order) and cov_vals_bar (n_bins, FP64).  values is the inverse's
+ * input.  Workspace: ggp_generate_vjp_workspace.  Precision as for ggp_generate_vjp. */
+int ggp_generate_inv_vjp(const void* points, int pdtype, int64_t n, int d, const int32_t* neighbors, int64_t n0,
+                         int k, const int32_t* indices, const double* cov_bins, const double* cov_vals, int n_bins,
+                         const void* values, int values_dtype, const void* xi_bar, int xi_bar_dtype,
+                         void* values_bar, int values_bar_dtype, double* cov_vals_bar, int precision,
+                         void* workspace, size_t workspace_bytes, void* stream);
+
+/* generate_logdet_vjp: reverse mode of generate_logdet -- what JAX autodiff of
+ * graphgp.generate_logdet (refine.py:234-240, 243-248, 251-275) returns for the cotangent
+ * *logdet_bar (a device FP64 scalar): cov_vals_bar (n_bins, FP64).  The seed block's part
+ * is (logdet_bar / 2) K^{-1} through its Cholesky factor (NaN when that factor fails).
+ * Workspace: ggp_generate_vjp_workspace. */
+int ggp_generate_logdet_vjp(const void* points, int pdtype, int64_t n, int d, const int32_t* neighbors, int64_t n0,
+                            int k, const double* cov_bins, const double* cov_vals, int n_bins,
+                            const double* logdet_bar, double* cov_vals_bar, int precision, void* workspace,
+                            size_t workspace_bytes, void* stream);
 
 /* refine: graphgp_cuda.refine(points, neighbors, offsets, cov_bins, cov_vals, initial_values,
  * xi), refine.py:119.  initial_values (n0), xi (n - n0) -> values (n, graph order) with the

@@ -788,4 +788,86 @@ void ora_refine_vjp(const double* pts, int64_t n, int d, const int32_t* nbrs, in
   for (int64_t i = 0; i < n0; ++i) ginit[i] = vbar[i];
 }
 
+// Reverse mode of refine_inv (mode 1, refine.py:198-231) or refine_logdet (mode 2,
+// refine.py:251-275) for fixed tables: every refined point on its own (no chain).  For point
+// i with w = A^{-1} b, z = A^{-1} u (u = values[nbrs]), sigma = L[k][k]:
+//   mode 1, cotangent t of xi_i = (v_i - b.z) / sigma:  gvalues[i] += t / sigma,
+//     gvalues[nbrs] += -t w / sigma, and the table sees al = -t / sigma (wbar = al u) and
+//     be = -t xi_i / (2 sigma^2) (the cotangent of sigma^2);
+//   mode 2, cotangent l of sum log sigma_i:  al = 0, be = l / (2 sigma^2);
+// dA_aa = -al w_a z_a + be w_a^2, dA_ab = -al (w_a z_b + w_b z_a) + 2 be w_a w_b,
+// db_a = al z_a - 2 be w_a, dc = be, each through its entry's lookup.  Written out by hand
+// from the reference's JAX autodiff; pinned by central differences in the tests.
+void ora_refine_local_vjp(const double* pts, int64_t n, int d, const int32_t* nbrs, int k, int64_t n0,
+                          const double* bins, const double* vals, int nbins, int mode, const double* values,
+                          const double* cot, double* gvalues, double* gvals) {
+  const int64_t n_ref = n - n0;
+  Table t{bins, vals, nbins};
+  for (int b = 0; b < nbins; ++b) gvals[b] = 0.0;
+  if (mode == 1)
+    for (int64_t i = 0; i < n; ++i) gvalues[i] = 0.0;
+  const int m = k + 1;
+  std::vector<double> K(m * m), Kc(m * m), w(k), z(k), tmp(k);
+  std::vector<int64_t> nb(k);
+  auto lookup_grad = [&](double r, double g) {
+    if (r < bins[0]) { gvals[0] += g; return; }
+    if (r > bins[nbins - 1]) { gvals[nbins - 1] += g; return; }
+    int i = (int)(std::upper_bound(bins, bins + nbins, r) - bins);
+    if (i < 1) i = 1;
+    if (i > nbins - 1) i = nbins - 1;
+    const double dx = bins[i] - bins[i - 1];
+    if (std::fabs(dx) <= 4.930380657631324e-32) { gvals[i - 1] += g; return; }
+    const double f = (r - bins[i - 1]) / dx;
+    gvals[i - 1] += g * (1.0 - f);
+    gvals[i] += g * f;
+  };
+  auto solve_a = [&](const double* rhs, double* out) {  // out = A^{-1} rhs with L_A
+    for (int a = 0; a < k; ++a) {
+      double acc = rhs[a];
+      for (int c = 0; c < a; ++c) acc -= K[a * m + c] * tmp[c];
+      tmp[a] = acc / K[a * m + a];
+    }
+    for (int a = k - 1; a >= 0; --a) {
+      double acc = tmp[a];
+      for (int c = a + 1; c < k; ++c) acc -= K[c * m + a] * out[c];
+      out[a] = acc / K[a * m + a];
+    }
+  };
+  std::vector<double> bvec(k), u(k);
+  for (int64_t r = 0; r < n_ref; ++r) {
+    const int64_t i = n0 + r;
+    for (int j = 0; j < k; ++j) nb[j] = nbrs[r * k + j];
+    joint_cov(pts, d, nb.data(), k, i, t, K.data());
+    Kc = K;
+    cholesky(K.data(), m);
+    for (int a = 0; a < k; ++a) bvec[a] = Kc[k * m + a];
+    solve_a(bvec.data(), w.data());
+    const double sigma = K[k * m + k];
+    double al = 0.0, be = 0.0;
+    for (int a = 0; a < k; ++a) z[a] = 0.0;
+    if (mode == 1) {
+      for (int j = 0; j < k; ++j) u[j] = values[nb[j]];
+      solve_a(u.data(), z.data());
+      double bz = 0.0;
+      for (int a = 0; a < k; ++a) bz += bvec[a] * z[a];
+      const double xo = (values[i] - bz) / sigma;
+      const double tt = cot[r];
+      al = -tt / sigma;
+      be = -tt * xo / (2.0 * sigma * sigma);
+      gvalues[i] += tt / sigma;
+      for (int j = 0; j < k; ++j) gvalues[nb[j]] += al * w[j];
+    } else {
+      be = cot[0] / (2.0 * sigma * sigma);
+    }
+    auto P = [&](int a) { return pts + (a < k ? nb[a] : i) * d; };
+    for (int a = 0; a < k; ++a) {
+      lookup_grad(pdist(P(a), P(a), d), -al * w[a] * z[a] + be * w[a] * w[a]);
+      for (int c = a + 1; c < k; ++c)
+        lookup_grad(pdist(P(a), P(c), d), -al * (w[a] * z[c] + w[c] * z[a]) + 2.0 * be * w[a] * w[c]);
+      lookup_grad(pdist(P(a), P(k), d), al * z[a] - 2.0 * be * w[a]);
+    }
+    lookup_grad(pdist(P(k), P(k), d), be);
+  }
+}
+
 }  // extern "C"

@@ -401,6 +401,98 @@ def refine_vjp(points, neighbors, covariance, values, xi, gvalues):
     return gxi, ginit, gvals
 
 
+def refine_inv_vjp(points, neighbors, covariance, values, xi_bar):
+    """Hand-written reverse mode of refine_inv's refined points (refine.py:198-231) for
+    fixed tables: (values_bar (N,), cov_vals_bar) for the cotangent xi_bar (N - n0,)."""
+    bins, vals = _cov(covariance)
+    p, nb, n, d, k, n0 = _graph_args(points, neighbors)
+    v, t = _f64(values), _f64(xi_bar)
+    gv = np.empty(n)
+    gvals = np.empty(len(bins))
+    lib().ora_refine_local_vjp(_ptr(p), _i64(n), ctypes.c_int(d), _ptr(nb), ctypes.c_int(k), _i64(n0), _ptr(bins),
+                               _ptr(vals), ctypes.c_int(len(bins)), ctypes.c_int(1), _ptr(v), _ptr(t), _ptr(gv),
+                               _ptr(gvals))
+    return gv, gvals
+
+
+def refine_logdet_vjp(points, neighbors, covariance, logdet_bar=1.0):
+    """Hand-written reverse mode of refine_logdet (refine.py:251-275): cov_vals_bar."""
+    bins, vals = _cov(covariance)
+    p, nb, n, d, k, n0 = _graph_args(points, neighbors)
+    lb = np.array([float(logdet_bar)])
+    gvals = np.empty(len(bins))
+    lib().ora_refine_local_vjp(_ptr(p), _i64(n), ctypes.c_int(d), _ptr(nb), ctypes.c_int(k), _i64(n0), _ptr(bins),
+                               _ptr(vals), ctypes.c_int(len(bins)), ctypes.c_int(2), None, _ptr(lb), None,
+                               _ptr(gvals))
+    return gvals
+
+
+def _chol_adjoint(L, Lbar):
+    """Cotangent of the symmetric K for the cotangent Lbar of L = chol(K) (Murray 2016, eq. 10):
+    1/2 L^{-T} Phi(L^T Lbar) L^{-1}, symmetrised."""
+    import scipy.linalg as sla
+
+    P = L.T @ Lbar
+    P = np.tril(P) - 0.5 * np.diag(np.diag(P))
+    S = sla.solve_triangular(L.T, sla.solve_triangular(L.T, P, lower=False).T, lower=False).T
+    return 0.5 * (S + S.T)
+
+
+def _pair_r(p):
+    return np.sqrt(((p[:, None, :] - p[None, :, :]) ** 2).sum(-1))
+
+
+def dense_inv_vjp(points, covariance, values, xi_bar):
+    """Reverse mode of generate_dense_inv (xi = L^{-1} values, refine.py:186-195) in NumPy
+    FP64: (values_bar, cov_vals_bar)."""
+    import scipy.linalg as sla
+
+    bins, _ = _cov(covariance)
+    p = _f64(points)
+    n = len(p)
+    _, L = generate_dense(p, covariance, np.zeros(n), return_L=True)
+    v, t = _f64(values), _f64(xi_bar)
+    xs = sla.solve_triangular(L, v, lower=True)
+    a = sla.solve_triangular(L.T, t, lower=False)
+    Kbar = _chol_adjoint(L, -np.tril(np.outer(a, xs)))
+    return a, _lookup_grad_all(bins, _pair_r(p).ravel(), Kbar.ravel())
+
+
+def dense_logdet_vjp(points, covariance, logdet_bar=1.0):
+    """Reverse mode of generate_dense_logdet (slogdet(K)[1] / 2, refine.py:243-248):
+    K_bar = logdet_bar K^{-1} / 2 through every pair's lookup."""
+    bins, _ = _cov(covariance)
+    p = _f64(points)
+    K = compute_cov_matrix(covariance, p, p)
+    Kbar = 0.5 * float(logdet_bar) * np.linalg.inv(K)
+    return _lookup_grad_all(bins, _pair_r(p).ravel(), Kbar.ravel())
+
+
+def generate_inv_vjp(graph, covariance, values, xi_bar):
+    """Reverse mode of generate_inv (refine.py:162-183): caller order in and out; returns
+    (values_bar (N,), cov_vals_bar (n_bins,))."""
+    n = len(graph.points)
+    n0 = n - len(graph.neighbors)
+    v, t = _f64(values), _f64(xi_bar)
+    if graph.indices is not None:
+        v, t = v[graph.indices], t[graph.indices]
+    gv, gtab = refine_inv_vjp(graph.points, graph.neighbors, covariance, v, t[n0:])
+    ga, gtd = dense_inv_vjp(graph.points[:n0], covariance, v[:n0], t[:n0])
+    gv[:n0] += ga
+    if graph.indices is not None:
+        out = np.empty_like(gv)
+        out[graph.indices] = gv
+        gv = out
+    return gv, gtab + gtd
+
+
+def generate_logdet_vjp(graph, covariance, logdet_bar=1.0):
+    """Reverse mode of generate_logdet (refine.py:234-240): cov_vals_bar (n_bins,)."""
+    n0 = len(graph.points) - len(graph.neighbors)
+    return (refine_logdet_vjp(graph.points, graph.neighbors, covariance, logdet_bar)
+            + dense_logdet_vjp(graph.points[:n0], covariance, logdet_bar))
+
+
 def dense_vjp(points, covariance, xi, gvalues):
     """Reverse mode of generate_dense (values = chol(K) @ xi) in NumPy FP64.
 

@@ -153,3 +153,154 @@ def test_grad_wrt_kernel_hyperparameters(kind):
     with torch.no_grad():
         fd = (loss(var0, len0 + h) - loss(var0, len0 - h)).item() / (2 * h)
     assert abs(gs.item() - fd) <= 1e-5 * max(abs(fd), 1.0), (gs.item(), fd)
+
+
+# ---------------------------------------------------------------- any k and d
+@pytest.mark.parametrize("n,d,n0,k,seed", [(3000, 3, 100, 40, 31), (2500, 6, 64, 10, 32), (3000, 4, 100, 20, 33),
+                                           (1500, 3, 60, 60, 34)])
+def test_vjp_generic_configurations_vs_oracle(n, d, n0, k, seed):
+    # k + 1 > 32 or d > 3: the one-lane-per-point adjoint (generate, inverse, log-det)
+    g, og, cov, ocov, xi, w = _setup(n, d, n0, k, seed)
+    v = gp.generate(g, cov, xi, precision="fp64")
+    gx, gv = gp.generate_vjp(g, cov, xi, v, w, precision="fp64")
+    ogx, ogv = O.generate_vjp(og, ocov, xi, w)
+    assert np.allclose(cpu(gx), ogx, rtol=1e-8, atol=1e-9 * np.abs(ogx).max())
+    assert np.allclose(cpu(gv), ogv, rtol=1e-7, atol=1e-8 * np.abs(ogv).max())
+    gvv, gvt = gp.generate_inv_vjp(g, cov, cpu(v), w, precision="fp64")
+    ogvv, ogvt = O.generate_inv_vjp(og, ocov, cpu(v), w)
+    assert np.allclose(cpu(gvv), ogvv, rtol=1e-8, atol=1e-9 * np.abs(ogvv).max())
+    assert np.allclose(cpu(gvt), ogvt, rtol=1e-7, atol=1e-8 * np.abs(ogvt).max())
+    glt = gp.generate_logdet_vjp(g, cov, 0.7, precision="fp64")
+    oglt = O.generate_logdet_vjp(og, ocov, 0.7)
+    assert np.allclose(cpu(glt), oglt, rtol=1e-7, atol=1e-8 * np.abs(oglt).max())
+
+
+# ---------------------------------------------------------------- inverse and log-det
+@pytest.mark.parametrize("n,d,n0,k,seed", [(3000, 3, 100, 8, 41), (5000, 3, 200, 20, 42), (4000, 2, 64, 12, 43),
+                                           (2500, 3, 300, 30, 44), (2000, 1, 16, 4, 45)])
+def test_inv_and_logdet_vjp_fp64_vs_oracle(n, d, n0, k, seed):
+    g, og, cov, ocov, xi, w = _setup(n, d, n0, k, seed)
+    values = O.generate(og, ocov, xi)
+    gvv, gvt = gp.generate_inv_vjp(g, cov, values, w, precision="fp64")
+    ogvv, ogvt = O.generate_inv_vjp(og, ocov, values, w)
+    assert np.allclose(cpu(gvv), ogvv, rtol=1e-8, atol=1e-9 * np.abs(ogvv).max())
+    assert np.allclose(cpu(gvt), ogvt, rtol=1e-7, atol=1e-8 * np.abs(ogvt).max())
+    glt = gp.generate_logdet_vjp(g, cov, precision="fp64")
+    oglt = O.generate_logdet_vjp(og, ocov)
+    assert np.allclose(cpu(glt), oglt, rtol=1e-7, atol=1e-8 * np.abs(oglt).max())
+
+
+def test_inv_vjp_dot_product_identity_and_fd():
+    # generate_inv is linear in the values: <t, J_inv u> = <J_inv^T t, u>; the table
+    # cotangents of the inverse and the log-det by central differences of the GPU forward
+    g, og, cov, ocov, xi, w = _setup(20000, 3, 500, 20, 47)
+    values = cpu(gp.generate(g, cov, xi, precision="fp64"))
+    gvv, gvt = gp.generate_inv_vjp(g, cov, values, w, precision="fp64")
+    rng = np.random.default_rng(48)
+    for _ in range(3):
+        u = rng.standard_normal(20000)
+        lhs = float(np.dot(w, cpu(gp.generate_inv(g, cov, u, precision="fp64"))))
+        assert np.isclose(lhs, float(np.dot(cpu(gvv), u)), rtol=1e-10)
+    bins, vals = cov
+    glt = cpu(gp.generate_logdet_vjp(g, cov, precision="fp64"))
+    gvt = cpu(gvt)
+    inv_loss = lambda vv: float(np.dot(w, cpu(gp.generate_inv(g, (bins, vv), values, precision="fp64"))))  # noqa
+    ld_loss = lambda vv: float(gp.generate_logdet(g, (bins, vv), precision="fp64"))  # noqa: E731
+    for grad, loss in ((gvt, inv_loss), (glt, ld_loss)):
+        nz = np.nonzero(np.abs(grad) > 1e-2 * np.abs(grad).max())[0]
+        for j in nz[:: max(1, len(nz) // 10)]:
+            h = 1e-8 * max(abs(float(vals[int(j)])), 1e-6)
+            e = torch.zeros_like(vals)
+            e[int(j)] = h
+            fd = (loss(vals + e) - loss(vals - e)) / (2 * h)
+            assert np.isclose(grad[j], fd, rtol=1e-4, atol=1e-6 * np.abs(grad).max()), (j, grad[j], fd)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp32_increment"])
+def test_inv_and_logdet_vjp_fp32_vs_oracle(precision):
+    # The inverse's and the log-det's cotangents scale as 1 / sigma and 1 / sigma^2, and
+    # sigma^2 = c - b.w loses digits to cancellation in FP32 where the local Schur complement
+    # is small, so the FP32 adjoints are compared on a well-conditioned field (3000 points,
+    # spacing comparable to the length scale); the FP64 tests above cover the rest.
+    g, og, cov, ocov, xi, w = _setup(3000, 3, 100, 12, 49)
+    values = O.generate(og, ocov, xi).astype(np.float32)
+    gvv, gvt = gp.generate_inv_vjp(g, cov, values, w.astype(np.float32), precision=precision)
+    ogvv, ogvt = O.generate_inv_vjp(og, ocov, values, w.astype(np.float32))
+    glt = gp.generate_logdet_vjp(g, cov, precision=precision)
+    oglt = O.generate_logdet_vjp(og, ocov)
+    ev = np.abs(cpu(gvv) - ogvv).max() / np.abs(ogvv).max()
+    et = np.linalg.norm(cpu(gvt) - ogvt) / np.linalg.norm(ogvt)
+    el = np.linalg.norm(cpu(glt) - oglt) / np.linalg.norm(oglt)
+    print(f"FP32 adjoints vs FP64 oracle: values_bar {ev:.2e}, inverse table {et:.2e}, log-det table {el:.2e}",
+          flush=True)
+    assert ev < 1e-3
+    assert et < 1e-2
+    assert el < 1e-2
+
+
+def test_autograd_inverse_and_logdet_likelihood():
+    # the fitting caller: a Gaussian negative log-likelihood of observed values,
+    # 0.5 |generate_inv(values)|^2 + generate_logdet, differentiated to the kernel
+    # hyperparameters through the tables (autograd), against central differences
+    rng = np.random.default_rng(51)
+    n = 6000
+    p = rng.uniform(0, 1, (n, 3)).astype(np.float32)
+    g = gp.build_graph(p, n0=200, k=12)
+    values = torch.as_tensor(rng.standard_normal(n), device="cuda")
+
+    def nll(v, s, vals_in=values):
+        cov = gp.extras.rbf_kernel(variance=v, scale=s, r_min=1e-5, r_max=2, n_bins=1000, jitter=1e-5)
+        xi = gp.generate_inv(g, cov, vals_in, precision="fp64")
+        return 0.5 * (xi * xi).sum() + gp.generate_logdet(g, cov, precision="fp64")
+
+    var0, len0 = 1.3, 0.15
+    v = torch.tensor(var0, dtype=torch.float64, device="cuda", requires_grad=True)
+    s = torch.tensor(len0, dtype=torch.float64, device="cuda", requires_grad=True)
+    vv = values.clone().requires_grad_(True)
+    L = nll(v, s, vv)
+    gv, gs, gval = torch.autograd.grad(L, [v, s, vv])
+    with torch.no_grad():
+        for name, g_, fd in (
+            ("variance", gv, (nll(var0 * (1 + 1e-6), len0) - nll(var0 * (1 - 1e-6), len0)).item() / (2e-6 * var0)),
+            ("scale", gs, (nll(var0, len0 * (1 + 1e-6)) - nll(var0, len0 * (1 - 1e-6))).item() / (2e-6 * len0)),
+        ):
+            assert abs(g_.item() - fd) <= 1e-5 * max(abs(fd), 1.0), (name, g_.item(), fd)
+        # with respect to the observed values: the gradient of 0.5 |xi|^2 is J_inv^T xi
+        xi = gp.generate_inv(g, gp.extras.rbf_kernel(variance=var0, scale=len0, r_min=1e-5, r_max=2, n_bins=1000,
+                                                     jitter=1e-5), values, precision="fp64")
+        ref, _ = gp.generate_inv_vjp(g, gp.extras.rbf_kernel(variance=var0, scale=len0, r_min=1e-5, r_max=2,
+                                                              n_bins=1000, jitter=1e-5), values, xi, precision="fp64")
+        assert torch.allclose(gval, ref, rtol=1e-10, atol=1e-12 * float(ref.abs().max()))
+
+
+def test_jjt_approaches_dense_reference_configuration():
+    # tests/test_refine.py:66-80: N = 1000 points (key 137), n0 = k = 200, matern p = 0,
+    # graph order without the permutation.  J = d generate / d xi, built here row by row
+    # from generate_vjp (row i = J^T e_i); J J^T must match the dense covariance within 0.02.
+    from helpers import ref_points
+
+    p = ref_points((1000, 3))
+    g0 = gp.build_graph(p, n0=200, k=200)
+    g = gp.Graph(g0.points, g0.neighbors, g0.offsets)
+    cov = gp.extras.matern_kernel(p=0, variance=1.0, cutoff=1.0, r_min=1e-4, r_max=10, n_bins=1000, jitter=1e-5)
+    n = 1000
+    zeros = torch.zeros(n, dtype=torch.float64, device="cuda")
+    v0 = gp.generate(g, cov, zeros, precision="fp64")
+    J = torch.empty((n, n), dtype=torch.float64, device="cuda")
+    for i in range(n):
+        e = torch.zeros(n, dtype=torch.float64, device="cuda")
+        e[i] = 1.0
+        J[i] = gp.generate_vjp(g, cov, zeros, v0, e, precision="fp64")[0]
+        if i % 100 == 0:
+            torch.cuda.synchronize()
+            print(f"J J^T: row {i}", flush=True)
+    dense = gp.compute_cov_matrix(cov, g.points, g.points)
+    err = float((J @ J.T - dense).abs().max())
+    print(f"max |J J^T - K| = {err:.3e}", flush=True)
+    assert err <= 0.02
+    # the rows are the forward pass's Jacobian: J u = generate(u)
+    u = torch.as_tensor(np.random.default_rng(61).standard_normal(n), device="cuda")
+    assert torch.allclose(J @ u, gp.generate(g, cov, u, precision="fp64"), rtol=1e-9, atol=1e-10)
+    xi = torch.as_tensor(ref_points((1000,)), device="cuda")
+    true_values = gp.generate_dense(g.points, cov, xi)
+    assert float((true_values - gp.generate(g, cov, xi, precision="fp64")).abs().max()) <= 0.02

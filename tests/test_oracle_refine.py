@@ -263,3 +263,57 @@ def test_dense_logdet_is_slogdet(n):
         q = np.concatenate([p, p[:1]])
         cov0 = O.rbf_kernel(variance=1.0, scale=0.2, r_min=1e-5, r_max=2.0, n_bins=1000, jitter=0.0)
         assert O.generate_dense_logdet(q, cov0) == -np.inf
+
+
+def _vjp_setup(seed=19, n=400, n0=30, k=8):
+    rng = np.random.default_rng(seed)
+    p = rng.uniform(0, 1, (n, 3))
+    g = O.build_graph(p, n0=n0, k=k)
+    cov = O.rbf_kernel(variance=1.0, scale=0.2, r_min=1e-4, r_max=2, n_bins=150, jitter=1e-3)
+    return rng, g, cov
+
+
+def test_generate_inv_vjp_pinned():
+    # generate_inv is linear in the values: <t, inv(u)> = <values_bar(t), u> exactly; the
+    # table cotangent is pinned by central differences of the whole inverse
+    rng, g, cov = _vjp_setup()
+    n = len(g.points)
+    values = O.generate(g, cov, rng.standard_normal(n))
+    t = rng.standard_normal(n)
+    gv, gtab = O.generate_inv_vjp(g, cov, values, t)
+    for _ in range(3):
+        u = rng.standard_normal(n)
+        assert np.isclose(float(t @ O.generate_inv(g, cov, u)), float(gv @ u), rtol=1e-10)
+    loss = lambda vals: float(t @ O.generate_inv(g, (cov[0], vals), values))  # noqa: E731
+    nz = np.nonzero(np.abs(gtab) > 1e-3 * np.abs(gtab).max())[0]
+    assert len(nz) > 8
+    for j in nz[:: max(1, len(nz) // 10)]:
+        h = 1e-7 * max(abs(cov[1][j]), 1e-3)
+        e = np.zeros_like(cov[1])
+        e[j] = h
+        fd = (loss(cov[1] + e) - loss(cov[1] - e)) / (2 * h)
+        assert np.isclose(gtab[j], fd, rtol=1e-4, atol=1e-6 * np.abs(gtab).max())
+
+
+def test_generate_logdet_vjp_pinned():
+    # central differences of the whole log-det (seed block's slogdet plus the refined sum)
+    _, g, cov = _vjp_setup(seed=23)
+    gtab = O.generate_logdet_vjp(g, cov, 1.7)
+    loss = lambda vals: 1.7 * O.generate_logdet(g, (cov[0], vals))  # noqa: E731
+    nz = np.nonzero(np.abs(gtab) > 1e-3 * np.abs(gtab).max())[0]
+    assert len(nz) > 8
+    for j in nz[:: max(1, len(nz) // 10)]:
+        h = 1e-7 * max(abs(cov[1][j]), 1e-3)
+        e = np.zeros_like(cov[1])
+        e[j] = h
+        fd = (loss(cov[1] + e) - loss(cov[1] - e)) / (2 * h)
+        assert np.isclose(gtab[j], fd, rtol=1e-4, atol=1e-6 * np.abs(gtab).max())
+    # the seed block alone: d/dK (log|det K| / 2) = K^{-1} / 2 (slogdet's derivative)
+    n0 = g.offsets[0]
+    gd = O.dense_logdet_vjp(g.points[:n0], cov)
+    dl = lambda vals: O.generate_dense_logdet(g.points[:n0], (cov[0], vals))  # noqa: E731
+    j = int(np.argmax(np.abs(gd)))
+    h = 1e-7 * abs(cov[1][j])
+    e = np.zeros_like(cov[1])
+    e[j] = h
+    assert np.isclose(gd[j], (dl(cov[1] + e) - dl(cov[1] - e)) / (2 * h), rtol=1e-5)
