@@ -65,6 +65,33 @@ __global__ void prep_table_kernel(const double* __restrict__ bins, const double*
   tab[j] = e;
 }
 
+// The packed FP32 table (refine_impl.hpp, TabP): nseg segments and the constant tail.
+__global__ void prep_packed_kernel(const double* __restrict__ bins, const double* __restrict__ vals, int nb,
+                                   int increment, TabP* __restrict__ tab) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nb) return;
+  const double f0 = vals[0];
+  TabP e;
+  e.pad = 0.f;
+  if (j == nb - 1) {
+    e.c = (float)(increment ? f0 - vals[nb - 1] : vals[nb - 1]);
+    e.s = 0.f;
+    e.e = (float)bins[nb - 1];
+  } else {
+    double fa = vals[j], fb = vals[j + 1];
+    if (increment) {
+      fa = f0 - fa;
+      fb = f0 - fb;
+    }
+    const double dx = bins[j + 1] - bins[j];
+    const double sl = (fabs(dx) <= 4.930380657631324e-32) ? 0.0 : (fb - fa) / dx;
+    e.c = (float)fa;
+    e.s = (float)sl;
+    e.e = (float)bins[j];
+  }
+  tab[j] = e;
+}
+
 __global__ void prep_meta_kernel(const double* __restrict__ bins, const double* __restrict__ vals, int nb,
                                  int increment, TabMeta* __restrict__ meta) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -81,9 +108,14 @@ __global__ void prep_meta_kernel(const double* __restrict__ bins, const double* 
     m.lb1 = (float)l1;
     m.inv_step = (float)((double)(nb - 2) / (l2 - l1));
     m.x1 = bins[1];
+    const double is = (double)(nb - 2) / (l2 - l1);
+    m.ka = (float)(0.5 * is);
+    m.kb = (float)(1.0 - l1 * is);
   } else {
     m.lb1 = 0.f;
     m.inv_step = 0.f;
+    m.ka = 0.f;
+    m.kb = 0.f;
     m.x1 = bins[nb > 1 ? 1 : 0];
   }
   m.log_exact = m.inv_step > 0.f ? 1 : 0;
@@ -99,12 +131,20 @@ __global__ void prep_check_kernel(const double* __restrict__ bins, int nb, TabMe
   const double mid = sqrt(bins[j] * bins[j + 1]);
   int g = log_guess((float)(mid * mid), lb1, inv_step);
   g = g < 1 ? 1 : (g > nb - 2 ? nb - 2 : g);
-  if (g != j || !(bins[j] > 0.0)) atomicAnd(&meta->log_exact, 0);
+  // the packed FP32 guess, at the same point
+  const int gp = packed_index((float)(mid * mid), meta->ka, meta->kb, nb - 1);
+  if (g != j || gp != j || !(bins[j] > 0.0)) atomicAnd(&meta->log_exact, 0);
 }
 
 }  // namespace
 
 size_t table_workspace(int nb, int t64) { return (size_t)nb * (t64 ? 24 : 12) + 512; }
+
+int prep_packed(const double* bins, const double* vals, int nb, int increment, TabP* tab, hipStream_t s) {
+  hipLaunchKernelGGL(prep_packed_kernel, dim3(grid_for(nb, 256)), dim3(256), 0, s, bins, vals, nb, increment, tab);
+  GGP_LAUNCH_CHECK();
+  return kOk;
+}
 
 int prep_table(const double* bins, const double* vals, int nb, int t64, int increment, void* tab, TabMeta* meta,
                hipStream_t s) {
@@ -359,6 +399,7 @@ size_t max_lds() {
 template <typename T>
 struct RefineWs {
   TabE<T>* tab = nullptr;
+  TabP* tabp = nullptr;  // FP32: the packed table of the register kernel
   TabMeta* meta = nullptr;
   TabE<double>* tab64 = nullptr;  // FP64 direct table for the FP64 recomputation (FP32 increment mode)
   TabMeta* meta64 = nullptr;
@@ -397,6 +438,7 @@ bool carve(Workspace& w, const RefineProblem& p, int mode, RefineWs<T>& b) {
   b.ctr = w.take<uint32_t>(4);  // first, so ggp_refine_counters finds it at offset 0
   b.tab = w.take<TabE<T>>(p.nb);  // nb-1 segments + nb edges (of T) fit in nb entries + nb/2
   w.take<T>(p.nb + 2);
+  if (sizeof(T) == 4) b.tabp = w.take<TabP>(p.nb);
   b.meta = w.take<TabMeta>(1);
   if (sizeof(T) == 4 && p.increment) {
     b.tab64 = w.take<TabE<double>>(p.nb);
@@ -454,6 +496,7 @@ void setup_args(RefineArgs<T>& a, const RefineProblem& p, RefineWs<T>& b) {
   a.n0 = p.n - p.n_ref;
   a.n_ref = p.n_ref;
   a.tab = b.tab;
+  a.tabp = b.tabp;
   a.meta = b.meta;
   a.ticket = b.ctr;
   a.flags = b.ctr + 1;
@@ -485,7 +528,9 @@ int run_kernel_inner(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hi
     GGP_LAUNCH_CHECK();
     return kOk;
   }
-  const size_t lds = (size_t)(p.nb - 1) * sizeof(TabE<T>) + (size_t)p.nb * sizeof(T);
+  // LDS: FP32 the packed table (nb entries), FP64 the segments and the edges
+  const size_t lds = sizeof(T) == 4 ? (size_t)p.nb * sizeof(TabP)
+                                    : (size_t)(p.nb - 1) * sizeof(TabE<T>) + (size_t)p.nb * sizeof(T);
   a.fix_lds_offset = (int)((lds + 15) & ~size_t(15));
   const int grid = device_cus() * 4;
   int rc = launch_refine<T>(a, kk, s, lds, grid);
@@ -571,6 +616,7 @@ int refine_flow(const RefineProblem& p, int mode, const RefineIO& io, void* ws, 
   const int64_t n0 = p.n - p.n_ref;
   GGP_TRY(fill_u32(b.ctr, 0u, 4, s));
   GGP_TRY(prep_table(p.bins, p.vals, p.nb, sizeof(T) == 8, p.increment, b.tab, b.meta, s));
+  if (b.tabp) GGP_TRY(prep_packed(p.bins, p.vals, p.nb, p.increment, b.tabp, s));
   if (b.tab64) GGP_TRY(prep_table(p.bins, p.vals, p.nb, 1, 0, b.tab64, b.meta64, s));
   RefineArgs<T> a{};
   a.mode = mode;

@@ -25,18 +25,59 @@ struct TabMeta {
   int nseg;
   double x_first, x_last, f_first, f_last, f_zero, x1;
   float lb1, inv_step;
-  int log_exact;  // 1: the log2 guess alone finds the segment (log-spaced bins)
+  float ka, kb;   // packed FP32 lookup: segment = clamp(trunc(log2(r^2) * ka + kb), 0, nseg)
+  int log_exact;  // 1: the log2 guesses alone find the segment (log-spaced bins)
 };
 
-// Segment guess used by the fast lookup; the table check in the prep kernel uses the
+// Segment guess used by the FP64 fast lookup; the table check in the prep kernel uses the
 // same expression, so "log_exact" certifies exactly what the kernel computes.
 __device__ __forceinline__ int log_guess(float r2, float lb1, float inv_step) {
   const float lr = 0.5f * __builtin_amdgcn_logf(r2);
   return 1 + (int)((lr - lb1) * inv_step);
 }
 
+// The FP32 table, packed: entry j < nseg is segment j on [e_j, e_{j+1}) with value
+// c + (r - e) s (c the value at e, so r - e is exact and the entry keeps its relative
+// precision); entry nseg is the constant tail {f_last, 0, x_last} for r >= x_last.  One
+// 16-byte LDS read returns a segment's line and edge.
+struct alignas(16) TabP {
+  float c, s, e, pad;
+};
+
+// Segment of r (given r^2) in a log-spaced packed table: t = 1 + (log2(r) - log2(x_1)) /
+// step folded into one FMA of log2(r^2); truncation equals floor wherever the clamp to
+// [0, nseg] does not decide (t < 1 means r < x_1: segment 0; r >= x_last: the tail).
+// The prep kernel certifies this very expression at every segment's middle.
+__device__ __forceinline__ int packed_index(float r2, float ka, float kb, int nseg) {
+  // clamped in FP32 (one v_med3), then truncated: -inf (r = 0) gives 0, NaN any valid index
+  return (int)__builtin_amdgcn_fmed3f(__builtin_amdgcn_logf(r2) * ka + kb, 0.0f, (float)nseg);
+}
+
+// Packed FP32 lookups: the log-spaced guess, and an exact binary search over the edges
+// (any monotone bins); both land on the tail entry for r >= x_last.
+__device__ __forceinline__ float packed_lookup_fast(const TabP* __restrict__ tab, float ka, float kb, int nseg,
+                                                    float r2) {
+  const TabP e = tab[packed_index(r2, ka, kb, nseg)];
+  return e.c + (__builtin_amdgcn_sqrtf(r2) - e.e) * e.s;
+}
+__device__ __forceinline__ float packed_lookup(const TabP* __restrict__ tab, int nseg, float r) {
+  int lo = 0, hi = nseg;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tab[mid].e <= r) lo = mid;
+    else hi = mid - 1;
+  }
+  const TabP e = tab[lo];
+  return e.c + (r - e.e) * e.s;
+}
+
 // kWeights: first phase of the two-phase generate (per-point weights and sigma, no waiting)
 enum RefineMode : int { kGenerate = 0, kInverse = 1, kLogdet = 2, kWeights = 3 };
+
+// Wavefronts of a workgroup that recompute flagged points in FP64, each with an LDS slab
+// (flagged points are about 1% of a tile's 64 points; two slabs keep the fused kernels at
+// two workgroups per CU with the 32 KB packed FP32 table)
+constexpr int kFixWaves = 2;
 
 template <typename T>
 struct RefineArgs {
@@ -54,6 +95,7 @@ struct RefineArgs {
   int out_f64;
   double* partials;  // logdet: one FP64 sum per tile
   const TabE<T>* tab;
+  const TabP* tabp;  // FP32: the packed table (nseg + 1 entries), which the FP32 kernel uses
   const TabMeta* meta;
   uint32_t* ticket;
   uint32_t* flags;  // [0] NaN produced, [1] malformed graph / wait limit
@@ -390,7 +432,9 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
   constexpr int S = P * R * (R + 1) / 2;
   constexpr int GPT = 256 / P;
   extern __shared__ __attribute__((aligned(16))) char smem_raw[];
+  constexpr bool PACKED = sizeof(T) == 4;  // FP32 reads the packed table
   TabE<T>* tab = reinterpret_cast<TabE<T>*>(smem_raw);
+  TabP* ptab = reinterpret_cast<TabP*>(smem_raw);
   __shared__ uint32_t s_tile;
   __shared__ double s_red[4];
   __shared__ int s_nfix;
@@ -401,15 +445,22 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
   // two variants are launched; the one that does not match the table exits at once
   if ((meta.log_exact != 0) != FAST) return;
   T* edges = reinterpret_cast<T*>(tab + meta.nseg);
-  const T* gedges = reinterpret_cast<const T*>(a.tab + meta.nseg);
-  for (int j = threadIdx.x; j < meta.nseg; j += blockDim.x) tab[j] = a.tab[j];
-  for (int j = threadIdx.x; j <= meta.nseg; j += blockDim.x) edges[j] = gedges[j];
+  if constexpr (PACKED) {
+    for (int j = threadIdx.x; j <= meta.nseg; j += blockDim.x) ptab[j] = a.tabp[j];
+  } else {
+    const T* gedges = reinterpret_cast<const T*>(a.tab + meta.nseg);
+    for (int j = threadIdx.x; j < meta.nseg; j += blockDim.x) tab[j] = a.tab[j];
+    for (int j = threadIdx.x; j <= meta.nseg; j += blockDim.x) edges[j] = gedges[j];
+  }
   __syncthreads();
   const T x1 = (T)meta.x1, x_last = (T)meta.x_last, f_last = (T)meta.f_last, f_zero = (T)meta.f_zero;
   const int nseg = meta.nseg;
-  const float lb1 = meta.lb1, inv_step = meta.inv_step;
-  auto lookup2 = [&](T r2) {
-    if constexpr (FAST) {
+  const float lb1 = meta.lb1, inv_step = meta.inv_step, ka = meta.ka, kb = meta.kb;
+  auto lookup2 = [&](T r2) -> T {
+    if constexpr (PACKED) {
+      if constexpr (FAST) return packed_lookup_fast(ptab, ka, kb, nseg, r2);
+      else return packed_lookup(ptab, nseg, sqrt(r2));
+    } else if constexpr (FAST) {
       const T r = fast_sqrt<T>(r2);
       return table_lookup_fast<T>(tab, edges, nseg, x1, x_last, f_last, lb1, inv_step, r, r2);
     } else {
@@ -423,9 +474,12 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
   const bool incr = a.increment != 0;
   const bool fixing = a.tab64 != nullptr;
   const T tau = (T)a.fix_tau;
-  double* slab = reinterpret_cast<double*>(smem_raw + a.fix_lds_offset) + (threadIdx.x >> 6) * (KK * (KK + 1));
+  // FP64 recomputation: the first kFixWaves wavefronts have an LDS slab each
+  double* slab = reinterpret_cast<double*>(smem_raw + a.fix_lds_offset) +
+                 ((threadIdx.x >> 6) % kFixWaves) * (KK * (KK + 1));
   // FP64 weights of the recomputed points: one KK-slot per group of the workgroup
-  double* wgrp = reinterpret_cast<double*>(smem_raw + a.fix_lds_offset) + 4 * (KK * (KK + 1)) + (threadIdx.x / P) * KK;
+  double* wgrp =
+      reinterpret_cast<double*>(smem_raw + a.fix_lds_offset) + kFixWaves * (KK * (KK + 1)) + (threadIdx.x / P) * KK;
   const int gbase = (threadIdx.x & 63) & ~(P - 1);
   const uint64_t gmask = (P == 64) ? ~0ull : ((1ull << P) - 1ull);
 
@@ -473,9 +527,12 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
 
     // ---- covariance entries (rows of this lane, all columns up to the slot width)
     T m[S];
-    T d0[R];  // original diagonal of this lane's rows (for the conditioning test)
+    // original diagonal of this lane's rows (for the conditioning test): the table at r = 0
+    // (f_zero), and in the increment formulation g(r_i0) + g(r_0i) - g(0) = 2 g(r_i0) for
+    // i > 0 (the increment table is exactly 0 at r = 0)
+    T d0[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) d0[r] = T(1);
+    for (int r = 0; r < R; ++r) d0[r] = (incr && P * r + lane != 0) ? g0[r] + g0[r] : f_zero;
     static_for<0, KK>([&](auto jc) {
       constexpr int j = decltype(jc)::value;
       constexpr int js = j % P, jr = j / P;
@@ -494,7 +551,6 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
             else v = (i == 0) ? -gj : (g0[r] + gj) - v;
           }
           m[P * r * (r + 1) / 2 + j] = v;
-          if (j >= P * r && j < P * r + P && i == j) d0[r] = v;
         }
       }
     });
@@ -509,7 +565,6 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
       constexpr int cs = col / P, cl = col % P;
       if (col < k) {
         const T dg = bcast<P, cl>(m[P * cs * (cs + 1) / 2 + col]);
-        if (fixing) bad = bad || !(dg >= tau * bcast<P, cl>(d0[cs]));
         const T rs = rsqrt_t<T>(dg);
         if (lane == cl) rdl[cs] = rs;
 #pragma unroll
@@ -526,6 +581,13 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
       }
     });
 
+    // a pivot that lost most of its diagonal (FP32 is not accurate there): pivot_i =
+    // 1 / rdl_i^2 < tau d0_i, tested by the lane that owns row i (a NaN pivot fails too)
+    if (fixing) {
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (P * r + lane < k) bad = bad || !(rdl[r] * rdl[r] * (tau * d0[r]) <= T(1));
+    }
     // Schur complement of the fine row: owned by lane k % P at slot k / P
     T skk = T(0);
 #pragma unroll
@@ -540,8 +602,8 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
 #pragma unroll
       for (int r = 0; r < R; ++r)
         if (P * r + lane == k) dk = d0[r];
-      const uint64_t bb = __ballot(owner && !(skk >= tau * dk));
-      flagged = valid && (bad || ((bb >> gbase) & gmask) != 0);
+      const uint64_t bb = __ballot((owner && !(skk >= tau * dk)) || bad);
+      flagged = valid && ((bb >> gbase) & gmask) != 0;
     }
     // FP64 recomputation of the tile's flagged points before any waiting: the flagged
     // groups are listed in LDS and the workgroup's four wavefronts take them in turn
@@ -552,9 +614,9 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
       if (flagged && lane == 0) s_fixlist[atomicAdd(&s_nfix, 1)] = grp;
       __syncthreads();
       const int nfix = s_nfix;
-      for (int it = threadIdx.x >> 6; it < nfix; it += 4) {
+      for (int it = threadIdx.x >> 6; it < nfix && (threadIdx.x >> 6) < kFixWaves; it += kFixWaves) {
         const int g = s_fixlist[it];
-        double* wslot = reinterpret_cast<double*>(smem_raw + a.fix_lds_offset) + 4 * (KK * (KK + 1)) + g * KK;
+        double* wslot = reinterpret_cast<double*>(smem_raw + a.fix_lds_offset) + kFixWaves * (KK * (KK + 1)) + g * KK;
         const double sg = fix64<T, D, FAST>(a, tile * GPT + g, mode != kLogdet, slab, KK + 1, wslot);
         if ((threadIdx.x & 63) == 0) {
           s_fixsig[g] = sg;

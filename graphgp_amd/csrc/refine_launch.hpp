@@ -33,10 +33,11 @@ int launch_config(RefineArgs<T> a, hipStream_t s, size_t lds_bytes, int grid) {
   a.ntiles = ((a.list_in ? (int64_t)a.fix_cap : a.n_ref) + gpt - 1) / gpt;
   if (a.ntiles == 0) return kOk;
   if ((int64_t)grid > a.ntiles) grid = (int)a.ntiles;
-  // the FP64 recomputation needs one KK x (KK+1) FP64 slab per wavefront and KK FP64
-  // weights per point group after the table
-  if (a.tab64)
-    lds_bytes = (size_t)a.fix_lds_offset + (size_t)4 * KK * (KK + 1) * sizeof(double) +
+  // the FP64 recomputation needs one KK x (KK+1) FP64 slab per recomputing wavefront and
+  // KK FP64 weights per point group after the table
+  // (the weights mode only lists the points; it needs no slabs)
+  if (a.tab64 && a.mode != kWeights)
+    lds_bytes = (size_t)a.fix_lds_offset + (size_t)kFixWaves * KK * (KK + 1) * sizeof(double) +
                 (size_t)(256 / P) * KK * sizeof(double);
   if (lds_bytes > refine_lds_limit()) {
     set_error("refine: covariance table does not fit in LDS (" + std::to_string(lds_bytes) + " bytes)");
