@@ -11,8 +11,9 @@
 // batch are close in that batch's storage too, so a cache line serves several gathers.
 //
 //   sched[q]   graph index (0-based over the refined points) of schedule slot q
-//   nbv[q, j]  neighbour j of sched[q] as a value slot: seeds keep their graph position,
-//              refined neighbour r moves to n0 + (slot of r)
+//   nbv[q, j]  the neighbours of sched[q] as value slots (seeds keep their graph position,
+//              refined neighbour r moves to n0 + (slot of r)): position 0 the nearest
+//              neighbour, positions 1 .. k-1 in increasing slot order
 //   pts_sched  the points in slot order (optional): with nbv as neighbours, the refinement
 //              kernel then works in slot space, its tiles' coordinate gathers local too
 #include "common.hpp"
@@ -103,6 +104,41 @@ __global__ void sched_nbrs_kernel(const int32_t* __restrict__ nbrs, int k, int64
   }
 }
 
+// Every row of nbv with its positions 1 .. k-1 in increasing slot order (position 0, the
+// nearest neighbour, stays first: the increment formulation is relative to it).  The
+// refinement kernels read row q's neighbours across the lanes of one point, and the rows of
+// a tile's points (close in space, in one batch) share many neighbours: in slot order the
+// same lane of neighbouring points reads the same or a nearby slot, so a gather instruction
+// touches fewer cache lines.  Rows of up to 64 entries, a bitonic network in registers.
+template <int W>
+__global__ void sort_rows_kernel(int32_t* __restrict__ nbv, int k, int64_t n_ref) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n_ref; q += (int64_t)gridDim.x * blockDim.x) {
+    int32_t* row = nbv + q * k;
+    int32_t v[W];
+#pragma unroll
+    for (int i = 0; i < W; ++i) v[i] = (i + 1 < k) ? row[i + 1] : 0x7fffffff;
+#pragma unroll
+    for (int size = 2; size <= W; size <<= 1)
+#pragma unroll
+      for (int stride = size >> 1; stride > 0; stride >>= 1)
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+          const int j = i ^ stride;
+          if (j > i) {
+            const bool up = (i & size) == 0;
+            const int32_t a = v[i], b = v[j];
+            if ((a > b) == up) {
+              v[i] = b;
+              v[j] = a;
+            }
+          }
+        }
+#pragma unroll
+    for (int i = 0; i < W; ++i)
+      if (i + 1 < k) row[i + 1] = v[i];
+  }
+}
+
 // points in value-slot order: seeds keep their place, refined slot q holds point sched[q]
 template <typename T>
 __global__ void sched_points_kernel(const T* __restrict__ pts, int d, int64_t n0, int64_t n,
@@ -173,6 +209,14 @@ int schedule_impl(const T* pts, int64_t n, int d, const int32_t* nbrs, int64_t n
   hipLaunchKernelGGL(sched_out_kernel, dim3(grid_capped(n_ref, 256)), dim3(256), 0, s, pay, n_ref, sched, b.slot);
   hipLaunchKernelGGL(sched_nbrs_kernel, dim3(grid_capped(n_ref * k, 256)), dim3(256), 0, s, nbrs, k, n0, n_ref,
                      (const int32_t*)sched, (const uint32_t*)b.slot, nbv);
+  if (k >= 3 && k <= 65) {
+    if (k <= 17)
+      hipLaunchKernelGGL(sort_rows_kernel<16>, dim3(grid_capped(n_ref, 256)), dim3(256), 0, s, nbv, k, n_ref);
+    else if (k <= 33)
+      hipLaunchKernelGGL(sort_rows_kernel<32>, dim3(grid_capped(n_ref, 256)), dim3(256), 0, s, nbv, k, n_ref);
+    else
+      hipLaunchKernelGGL(sort_rows_kernel<64>, dim3(grid_capped(n_ref, 256)), dim3(256), 0, s, nbv, k, n_ref);
+  }
   if (pts_sched)
     hipLaunchKernelGGL(sched_points_kernel<T>, dim3(grid_capped(n * d, 256)), dim3(256), 0, s, pts, d, n0, n,
                        (const int32_t*)sched, pts_sched);

@@ -487,8 +487,10 @@ def test_dense_seed_block_vs_oracle(n):
 
 
 def test_scheduled_generate_equals_unscheduled(monkeypatch):
-    # the dataflow schedule changes only the order of evaluation and storage: bit-identical
-    # values; the schedule is a permutation that keeps the batch order
+    # the dataflow schedule changes the order of evaluation and storage, and lists each
+    # point's neighbours (after the nearest) in slot order: the local systems are the same
+    # up to a permutation, so the values agree to rounding; the schedule is a permutation
+    # that keeps the batch order
     import sys
 
     R = sys.modules["graphgp_amd.graph"]
@@ -501,8 +503,10 @@ def test_scheduled_generate_equals_unscheduled(monkeypatch):
     monkeypatch.setattr(R, "SCHEDULE_MIN_REFINED", 0)
     b = gp.generate(g, cov, xi, precision="fp32_increment")
     c = gp.generate(g, cov, xi, precision="fp32_increment")  # cached schedule
-    assert torch.equal(a.view(torch.int32), b.view(torch.int32))
-    assert torch.equal(a.view(torch.int32), c.view(torch.int32))
+    d64 = float((a - b).abs().max())
+    print(f"fp32_increment scheduled vs unscheduled: max |diff| {d64:.3e}", flush=True)
+    assert d64 <= 1e-4  # the mode's accuracy bound against FP64
+    assert torch.equal(b.view(torch.int32), c.view(torch.int32))
     order, nbv, pts_s = g._ggp_schedule[1]
     o = cpu(order).astype(np.int64)
     n0 = g.offsets[0]
@@ -513,6 +517,7 @@ def test_scheduled_generate_equals_unscheduled(monkeypatch):
     slot[o] = np.arange(len(o))
     nb = cpu(g.neighbors).astype(np.int64)[o]
     expect = np.where(nb < n0, nb, n0 + slot[np.maximum(nb - n0, 0)])
+    expect[:, 1:] = np.sort(expect[:, 1:], axis=1)
     assert np.array_equal(cpu(nbv), expect)
     P = cpu(g.points)
     assert np.array_equal(cpu(pts_s), np.concatenate([P[:n0], P[n0 + o]]))
@@ -520,14 +525,21 @@ def test_scheduled_generate_equals_unscheduled(monkeypatch):
     monkeypatch.setattr(R, "SCHEDULE_MIN_REFINED", 1 << 40)
     ref = {pr: gp.generate(g, cov, xi, precision=pr) for pr in ("fp32", "fp64")}
     monkeypatch.setattr(R, "SCHEDULE_MIN_REFINED", 0)
-    for pr in ("fp32", "fp64"):
+    for pr, tol in (("fp32", 2e-2), ("fp64", 1e-9)):
         v = gp.generate(g, cov, xi, precision=pr)
-        assert torch.equal(v, ref[pr]), pr
+        dv = float((v - ref[pr]).abs().max())
+        print(f"{pr} scheduled vs unscheduled: max |diff| {dv:.3e}", flush=True)
+        assert dv <= tol, (pr, dv)
     # points changed in place: the cached schedule (slot-order copies) must not be reused
+    old = g._ggp_schedule[1]
     with torch.no_grad():
         g.points[n0 + 7] += 1e-3
         g.points[-1] -= 1e-3
     a = gp.generate(g, cov, xi, precision="fp32_increment")
+    assert g._ggp_schedule[1] is not old
+    P = cpu(g.points)
+    o2 = cpu(g._ggp_schedule[1][0]).astype(np.int64)
+    assert np.array_equal(cpu(g._ggp_schedule[1][2]), np.concatenate([P[:n0], P[n0 + o2]]))
     monkeypatch.setattr(R, "SCHEDULE_MIN_REFINED", 1 << 40)
     b = gp.generate(g, cov, xi, precision="fp32_increment")
-    assert torch.equal(a.view(torch.int32), b.view(torch.int32))
+    assert float((a - b).abs().max()) <= 1e-4
