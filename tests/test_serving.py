@@ -72,7 +72,7 @@ def test_result_alone_equals_result_in_mix():
     srv.close()
     assert stats["requests"] == len(reqs)
     for r in reqs:
-        g = gp.build_graph(r.points, n0=r.n0, k=r.k)
+        g = gp.build_graph(r.points, n0=r.n0, k=r.k, schedule=False)  # as the server builds
         fn = gp.generate if r.op == "generate" else gp.generate_inv
         alone = fn(g, r.covariance, r.data, precision=r.precision)
         assert alone.dtype == r.result.dtype == torch.float32
@@ -128,7 +128,7 @@ def test_large_and_small_requests_alone_equal_in_mix():
     srv.run(reqs, realtime=False)
     srv.close()
     for r in reqs:
-        g = gp.build_graph(r.points, n0=r.n0, k=r.k)
+        g = gp.build_graph(r.points, n0=r.n0, k=r.k, schedule=False)  # as the server builds
         fn = gp.generate if r.op == "generate" else gp.generate_inv
         alone = fn(g, r.covariance, r.data, precision=r.precision)
         assert torch.equal(alone.view(torch.int32), r.result.view(torch.int32)), (r.rid, r.op, r.n)
