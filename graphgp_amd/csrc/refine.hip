@@ -404,6 +404,7 @@ struct RefineWs {
   TabE<double>* tab64 = nullptr;  // FP64 direct table for the FP64 recomputation (FP32 increment mode)
   TabMeta* meta64 = nullptr;
   uint32_t* ctr = nullptr;  // [0] ticket, [1] NaN flag, [2] error flag, [3] FP64-recomputed points
+  uint32_t* gtk = nullptr;  // the dataflow kernel's group tickets, one cache line each
   double* partials = nullptr;
   T* vals = nullptr;  // graph-order values (N)
   double* dense = nullptr;
@@ -455,6 +456,7 @@ bool carve(Workspace& w, const RefineProblem& p, int mode, RefineWs<T>& b) {
   }
   if (generic_config(p)) b.scratch = w.take<T>(generic_scratch_bytes<T>(p.k) / sizeof(T));
   if (two_phase<T>(p, mode)) {
+    b.gtk = w.take<uint32_t>(kMaxGroups * 32);
     b.fix_cap = (uint32_t)std::min<int64_t>(fix_capacity(p.n_ref), p.n_ref);
     b.wts = w.take<T>((size_t)p.n_ref * p.k);
     b.sig = w.take<T>(p.n_ref);
@@ -595,7 +597,11 @@ int run_two_phase(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hipSt
   auto kern = apply_kernel<T, P, RM>;
   const int grid = (int)std::min<int64_t>(a.ntiles, persistent_grid((const void*)kern, cus * 8, 0));
   a.yield = yield_cfg().dev;
-  a.keep = yield_keep(grid);
+  a.groups = std::min(grid, kMaxGroups);
+  // every group keeps a workgroup that does not yield (the first `keep` ones do not)
+  a.keep = std::max<uint32_t>(yield_keep(grid), (uint32_t)a.groups);
+  a.group_ticket = b.gtk;
+  GGP_TRY(fill_u32(b.gtk, 0u, kMaxGroups * 32, s));
   hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, s, a);
   GGP_LAUNCH_CHECK();
   return kOk;

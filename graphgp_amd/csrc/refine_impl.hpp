@@ -79,6 +79,10 @@ enum RefineMode : int { kGenerate = 0, kInverse = 1, kLogdet = 2, kWeights = 3 }
 // two workgroups per CU with the 32 KB packed FP32 table)
 constexpr int kFixWaves = 2;
 
+// consecutive tiles one XCD group of the dataflow kernel takes at a time
+constexpr uint32_t kGroupTiles = 8;
+constexpr int kMaxGroups = 8;  // XCDs of an MI355X
+
 template <typename T>
 struct RefineArgs {
   const void* pts;
@@ -128,6 +132,15 @@ struct RefineArgs {
   // yield: while *yield is nonzero, workgroups from `keep` on stop taking tiles
   const uint32_t* yield;
   uint32_t keep;
+  // XCD-aware tiles of the dataflow kernel: workgroup b belongs to group b % groups (the
+  // device places consecutive workgroups on its XCDs in turn), and group g takes blocks of
+  // kGroupTiles consecutive tiles, blocks g, g + groups, ..., from its own counter
+  // (group_ticket[g * 32], one cache line each), so neighbouring tiles, which share
+  // neighbour values, run behind one L2.  Every group takes its tiles in increasing order,
+  // which keeps the dataflow's progress argument (the lowest unfinished tile is always
+  // taken, and its dependencies are finished).
+  uint32_t* group_ticket;
+  int groups;
   // dataflow over stored weights in schedule order (or null): slot q takes refined point
   // sched[q], reads its weights and sigma at slot q (the weights phase then ran in slot
   // space), its neighbours' values at the slots nbv[q, .], and stores its value at slot
@@ -847,8 +860,17 @@ __global__ __launch_bounds__(256) void apply_kernel(RefineArgs<T> a) {
   const bool incr = a.increment != 0;
   const int gbase = (threadIdx.x & 63) & ~(P - 1);
   const uint64_t gmask = (P == 64) ? ~0ull : ((1ull << P) - 1ull);
+  const uint32_t group = blockIdx.x % (uint32_t)a.groups;
   for (;;) {
-    if (threadIdx.x == 0) s_tile = yield_now(a.yield, a.keep) ? 0xffffffffu : atomicAdd(a.ticket, 1u);
+    if (threadIdx.x == 0) {
+      if (yield_now(a.yield, a.keep)) {
+        s_tile = 0xffffffffu;
+      } else {
+        const uint32_t t = atomicAdd(&a.group_ticket[group * 32], 1u);
+        const uint64_t tl = ((uint64_t)(t / kGroupTiles) * a.groups + group) * kGroupTiles + t % kGroupTiles;
+        s_tile = tl < 0xffffffffull ? (uint32_t)tl : 0xffffffffu;
+      }
+    }
     __syncthreads();
     const int64_t tile = s_tile;
     __syncthreads();
