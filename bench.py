@@ -66,6 +66,61 @@ def algorithmic_flops(k, d):
     return pairs * (3 * d) + pairs * 3 + k ** 3 / 3 + 2 * k * k + 4 * k
 
 
+# Algorithmic bytes per unit of each timed stage (the unit is a refined point unless noted):
+# the data the stage must read or write once, with neighbour coordinates and values counted
+# per gather.  The weights phase reads the k + 1 points' coordinates and the k indices and
+# writes k weights, sigma and the recomputation slot; the FP64 list (per recomputed point)
+# reads the same and writes k FP64 weights and sigma; the dataflow reads indices, weights,
+# slot, schedule, permutation, xi, sigma and the k neighbour values and writes the value and
+# the output.  Build stages (per point): the tree reads the points and writes them with the
+# permutation and split dimensions; the kNN reads its query and writes k indices; depths
+# read the k indices and write a depth; the ordering reads and writes points, index,
+# neighbours and depth; the schedule (per refined point) reads indices and coordinates and
+# writes them in slot order with the slot.
+def stage_bytes(stage, k, d):
+    return {
+        "weights": (k + 1) * d * 4 + 4 * k + 4 * k + 8,
+        "fp64_list": (k + 1) * d * 4 + 4 * k + 4 + 8 * k + 8,
+        "dataflow": 12 * k + 28,
+        "refine": algorithmic_bytes(k, d, 4, 8),
+        "tree": 2 * d * 4 + 8,
+        "knn": d * 4 + 4 * k,
+        "depths": 4 * k + 4,
+        "order": 2 * (d * 4 + 4 + 4 * k + 4),
+        "schedule": 2 * (4 * k + d * 4) + 4,
+    }.get(stage)
+
+
+def stage_report(times, steps, units, flops, peak_tflops, k, d):
+    """Per-stage mean device ms, algorithmic GB/s (and fraction of HBM peak) and, where the
+    stage does floating-point work, TFLOP/s and its fraction of the vector peak."""
+    out = {}
+    for name, (ms, cnt) in times.items():
+        m = ms / steps
+        e = {"ms": round(m, 4)}
+        b = stage_bytes(name, k, d)
+        u = units.get(name)
+        if b is not None and u and m > 0:
+            gbs = b * u / (m * 1e-3) / 1e9
+            e.update(bytes_per_unit=b, units=int(u), hbm_GBs=round(gbs, 1), hbm_frac=round(gbs / PEAK_HBM_GBS, 4))
+        f = flops.get(name)
+        if f and m > 0:
+            tf = f / (m * 1e-3) / 1e12
+            e.update(tflops=round(tf, 3), valu_frac=round(tf / peak_tflops[name], 4))
+        out[name] = e
+    return out
+
+
+def cpu_quota():
+    """CPUs this process may use by its cgroup quota (cpu.max), or None without one."""
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, per = f.read().split()[:2]
+        return None if q == "max" else round(int(q) / int(per), 2)
+    except (OSError, ValueError):
+        return None
+
+
 def run_differentiation(args):
     """The differentiation run: N=4,194,304 uniform in [0,1]^3, n0=500, k=20, RBF (variance 1,
     scale 0.1) on r=0 plus 1000 log bins over [1e-5, 2], jitter 1e-5.  L = sum w_i values_i
@@ -329,6 +384,14 @@ def main():
     build_s = time.time() - t
     log(f"build_graph {build_s * 1e3:.1f} ms, {len(g.offsets) - 1} batches")
     n_ref = n - n0
+    # build stages: one more build (and schedule) with the library's stage timing on
+    _lib.stage_profile(True)
+    g2 = gp.build_graph(pts_d, n0=n0, k=k)
+    build_stages = stage_report(_lib.stage_times(), 1, {"tree": n, "knn": n_ref, "depths": n_ref, "order": n,
+                                                        "schedule": n_ref}, {}, {}, k, d)
+    _lib.stage_profile(False)
+    del g2
+    log(f"build stages: {build_stages}")
 
     start_ev = torch.cuda.Event(enable_timing=True)
     stop_ev = torch.cuda.Event(enable_timing=True)
@@ -340,6 +403,7 @@ def main():
             out = gp.generate(g, cov, xi_d, precision=precision)
         torch.cuda.synchronize()
         kms = []
+        _lib.stage_profile(True)  # events around every stage of each step, on the call's stream
         t0 = time.time()
         for _ in range(steps):
             ks, ke = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -351,14 +415,17 @@ def main():
             kms.append((ks, ke))
         torch.cuda.synchronize()
         wall = (time.time() - t0) / steps
+        stages = _lib.stage_times()
+        _lib.stage_profile(False)
         kernel_ms = float(np.mean([a.elapsed_time(b) for a, b in kms]))
-        return out, wall, kernel_ms
+        return out, wall, kernel_ms, stages
 
     results = {}
     for prec, steps in (("fp32_increment", args.steps), ("fp64", max(2, args.steps // 2)),
                         ("fp32", max(2, args.steps // 2))):
-        out, wall, kms = timed(prec, steps, args.warmup)
-        results[prec] = dict(out=out, ms=wall * 1e3, kernel_ms=kms, pts_per_s=n_ref / wall)
+        out, wall, kms, stages = timed(prec, steps, args.warmup)
+        results[prec] = dict(out=out, ms=wall * 1e3, kernel_ms=kms, pts_per_s=n_ref / wall, stages=stages,
+                             steps=steps)
         log(f"{prec}: {wall * 1e3:.2f} ms/step, dominant refine kernel {kms:.2f} ms, {n_ref / wall / 1e9:.3f} G pts/s")
 
     # FP64 path checked point by point against the oracle (given its neighbours' FP64 values)
@@ -389,7 +456,36 @@ def main():
         dx = np.abs(xb - xi)
         errs[prec]["local_xi_max_abs"] = float(np.nanmax(dx))
         errs[prec]["local_xi_rms"] = float(np.sqrt(np.nanmean(dx ** 2)))
+    # log-det of each FP32 path against the FP64 one
+    ld64 = float(gp.generate_logdet(g, cov, precision="fp64"))
+    for prec in ("fp32_increment", "fp32"):
+        ld = float(gp.generate_logdet(g, cov, precision=prec))
+        errs[prec]["logdet_abs"] = abs(ld - ld64)
+        errs[prec]["logdet_rel"] = abs(ld - ld64) / abs(ld64)
+    errs["logdet_fp64"] = ld64
     log(f"fp64 vs oracle (sampled, local): {loc_err:.3e}; errors vs fp64: {errs}")
+
+    # every stage of the timed steps: device ms, algorithmic GB/s, TFLOP/s where it computes
+    n_fix = cnt["fp64_recomputed"]
+    units = {"weights": n_ref, "fp64_list": n_fix, "dataflow": n_ref, "refine": n_ref}
+    fl = algorithmic_flops(k, d)
+    stage_lines = {}
+    for prec, peak in (("fp32_increment", PEAK_FP32_TFLOPS), ("fp64", PEAK_FP64_TFLOPS), ("fp32", PEAK_FP32_TFLOPS)):
+        rr = results[prec]
+        flops_ = {"weights": fl * n_ref, "fp64_list": fl * n_fix, "dataflow": (2 * k + 4) * n_ref,
+                  "refine": fl * n_ref, "seed_block": n0 ** 3 / 3 + 2 * n0 ** 2}
+        peaks = {"weights": PEAK_FP32_TFLOPS, "fp64_list": PEAK_FP64_TFLOPS, "dataflow": PEAK_FP32_TFLOPS,
+                 "refine": peak, "seed_block": PEAK_FP64_TFLOPS}
+        by = dict(units)
+        if prec == "fp32":
+            by["refine"] = None  # FP32 fused kernel: its own byte count below
+        stage_lines[prec] = stage_report(rr["stages"], rr["steps"], by, flops_, peaks, k, d)
+        if prec == "fp32" and "refine" in stage_lines[prec]:
+            e = stage_lines[prec]["refine"]
+            b = algorithmic_bytes(k, d, 4, 4)
+            gbs = b * n_ref / (e["ms"] * 1e-3) / 1e9
+            e.update(bytes_per_unit=b, units=n_ref, hbm_GBs=round(gbs, 1), hbm_frac=round(gbs / PEAK_HBM_GBS, 4))
+    log(f"stages: {stage_lines}")
 
     # roofline of the dominant kernel of the FP32-increment generate, its weights phase
     # (VALU bound: no MFMA in this kernel); the library's events bracket exactly that launch.
@@ -426,8 +522,7 @@ def main():
 
     cpu_baseline = None
     if not args.no_cpu:
-        cores = len(os.sched_getaffinity(0))
-        cores = min(cores, 16)
+        cores = len(os.sched_getaffinity(0))  # every CPU this process may run on
         O.set_threads(cores)
         opts = gpts.astype(np.float64)
         # calibrate, then a bounded sample of about args.cpu_seconds
@@ -443,6 +538,7 @@ def main():
             "value": round(m / dt, 1),
             "unit": "refined points/s",
             "cores": cores,
+            "cpu_quota": cpu_quota(),
             "kind": "port",
             "sample": f"first {m} refined points of the same graph (FP64 oracle restatement of the fast_jit "
                       f"refinement, refine.py:123-145), {dt:.1f} s",
@@ -483,6 +579,8 @@ def main():
         "fp32_increment_fp64_recomputed_fraction": fix_frac,
         "fp64_vs_oracle_sampled_max_abs": loc_err,
         "build_graph_ms": round(build_s * 1e3, 1),
+        "build_stages": build_stages,
+        "stages": stage_lines,
         "roofline": roofline,
         "cpu_baseline": cpu_baseline,
         "serve": serve,

@@ -70,6 +70,9 @@ _SIGS = {
     "ggp_generate_logdet_noalloc": (_int, [_p, _int, _i64, _int, _p, _i64, _int, _p, _p, _int, _p, _int, _p]),
     "ggp_cov_matrix": (_int, [_p, _i64, _p, _i64, _int, _int, _p, _p, _int, _p, _p]),
     "ggp_profile_events": (_int, [_p, _p]),
+    "ggp_stage_profile": (_int, [_int]),
+    "ggp_stage_times": (_int, [_int, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_double),
+                            ctypes.POINTER(ctypes.c_int)]),
     "ggp_refine_counters": (_int, [_p, _u32p, _p]),
     "ggp_set_cu_budget": (_int, [_int]),
     "ggp_set_yield": (_int, [_p, _int]),
@@ -133,6 +136,24 @@ def to_host(t):
     h.copy_(t, non_blocking=True)
     torch.cuda.current_stream().synchronize()
     return h
+
+
+def stage_profile(enable: bool) -> None:
+    """Switch the library's stage timing on (clearing its sums) or off, for this thread."""
+    check(lib().ggp_stage_profile(1 if enable else 0), "stage_profile")
+
+
+def stage_times() -> dict:
+    """{stage name: (total device ms, occurrences)} recorded since stage_profile(True)."""
+    L = lib()
+    n = L.ggp_stage_times(0, None, None, None)
+    check(min(n, 0), "stage_times")
+    names = (ctypes.c_char_p * max(n, 1))()
+    ms = (ctypes.c_double * max(n, 1))()
+    cnt = (ctypes.c_int * max(n, 1))()
+    n = L.ggp_stage_times(n, names, ms, cnt)
+    check(min(n, 0), "stage_times")
+    return {names[i].decode(): (ms[i], cnt[i]) for i in range(n)}
 
 
 def workspace(nbytes: int):

@@ -1,4 +1,5 @@
 // abi.hip -- the extern "C" entry points declared in include/graphgp_amd.h.
+#include <optional>
 #include <string>
 
 #include "../../include/graphgp_amd.h"
@@ -236,9 +237,13 @@ int ggp_build_graph(const void* points, int64_t n, int d, int dtype, int64_t n0,
     return kWorkspaceTooSmall;
   }
   hipStream_t s = S(stream);
+  std::optional<StageScope> stage;
+  stage.emplace("tree", s);
   GGP_TRY(build_tree(points, n, d, dtype, g.tpts, g.split, g.idx, g.sub, g.sub_bytes, s));
+  stage.emplace("knn", s);
   GGP_TRY(query_neighbors(g.tpts, g.split, n, d, dtype, nullptr, nullptr, n0, n - n0, k, g.nbrs, g.sub, g.sub_bytes,
                           s));
+  stage.emplace("depths", s);
   GGP_TRY(compute_depths(g.nbrs, n - n0, k, n0, g.depths, g.sub, g.sub_bytes, s));
   uint32_t status = 0;
   GGP_TRY(ggp_compute_depths_status(g.sub, &status, stream));
@@ -246,6 +251,7 @@ int ggp_build_graph(const void* points, int64_t n, int d, int dtype, int64_t n0,
     set_error("build_graph: neighbour graph is not in topological order (internal error)");
     return kHipError;
   }
+  stage.emplace("order", s);
   GGP_TRY(fill_u32(g.dmax, 0u, 1, s));
   hipLaunchKernelGGL(max_kernel, dim3(std::min<int64_t>(grid_for(n, 256), 1024)), dim3(256), 0, s, g.depths, n,
                      g.dmax);
@@ -319,6 +325,7 @@ int ggp_graph_schedule(const void* points, int pdtype, int64_t n, int d, const i
                        const int64_t* offsets, int64_t n_offsets, int32_t* schedule, int32_t* schedule_neighbors,
                        void* schedule_points, void* workspace, size_t workspace_bytes, void* stream) {
   GGP_TRY(check_dtype(pdtype, "graph_schedule points"));
+  StageScope stage("schedule", S(stream));
   return graph_schedule(points, pdtype == GGP_F64 ? 1 : 0, n, d, neighbors, n0, k, offsets, n_offsets, schedule,
                         schedule_neighbors, schedule_points, workspace, workspace_bytes, S(stream));
 }

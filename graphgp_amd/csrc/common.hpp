@@ -57,6 +57,23 @@ const char* last_error();
     if (s_ != 0) return s_;      \
   } while (0)
 
+// ------------------------------------------------------------------ stage timing
+// While stage timing is on for the calling host thread (ggp_stage_profile), a StageScope
+// records one event where it is constructed and one where it is destroyed, on stream s,
+// so that ggp_stage_times can report the stage's device time; otherwise it does nothing.
+// `name` must be a string literal (it is kept by pointer).
+class StageScope {
+ public:
+  StageScope(const char* name, hipStream_t s);
+  ~StageScope();
+  StageScope(const StageScope&) = delete;
+  StageScope& operator=(const StageScope&) = delete;
+
+ private:
+  int slot_;
+  hipStream_t stream_;
+};
+
 // ------------------------------------------------------------------ workspace
 // A bump allocator over a caller-provided device buffer (the library never
 // allocates).  With base == nullptr it only measures.

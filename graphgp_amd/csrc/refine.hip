@@ -29,6 +29,8 @@
 //    relative to the nearest neighbour, B_ij = g(r_i0) + g(r_0j) - g(r_ij) with the
 //    variogram table g = f_0 - f, so that the entries keep their relative precision
 //    when k(r) ~ k(0) (clustered points); the value is v_0 + mean(increment) + std*xi.
+#include <optional>
+
 #include "common.hpp"
 #include "graphgp_amd_internal.hpp"
 #include "refine_impl.hpp"
@@ -547,6 +549,8 @@ template <typename T>
 int run_two_phase(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hipStream_t s) {
   if (g_prof_start) GGP_HIP(hipEventRecord(g_prof_start, s));
   setup_args<T>(a, p, b);  // the later phases launch with this copy of the arguments
+  std::optional<StageScope> stage;
+  stage.emplace("weights", s);
   // with a schedule, the weights and FP64 phases work in slot space (points in slot order,
   // neighbours as value slots), so their weights, sigma and list are slot-indexed
   const void* w_pts = p.sched ? p.sched_pts : p.pts;
@@ -558,6 +562,7 @@ int run_two_phase(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hipSt
   GGP_TRY(run_kernel_inner<T>(a, pw, b, s));
   // the profiling events bracket the dominant kernel: the weights phase
   if (g_prof_stop) GGP_HIP(hipEventRecord(g_prof_stop, s));
+  stage.emplace("fp64_list", s);
   // FP64 recomputation of the listed points with the FP64 register kernel (weights mode over
   // the list, FP64 direct table)
   {
@@ -586,6 +591,7 @@ int run_two_phase(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hipSt
     GGP_TRY(launch_refine<double>(d, p.k + 1, s, lds64, device_cus() * 4));
   }
   const int cus = device_cus();
+  stage.emplace("dataflow", s);
   GGP_TRY(fill_u32(b.ctr, 0u, 1, s));  // fresh ticket for the dataflow
   a.mode = kGenerate;
   a.pts = p.pts;
@@ -620,16 +626,20 @@ int refine_flow(const RefineProblem& p, int mode, const RefineIO& io, void* ws, 
     return kWorkspaceTooSmall;
   }
   const int64_t n0 = p.n - p.n_ref;
-  GGP_TRY(fill_u32(b.ctr, 0u, 4, s));
-  GGP_TRY(prep_table(p.bins, p.vals, p.nb, sizeof(T) == 8, p.increment, b.tab, b.meta, s));
-  if (b.tabp) GGP_TRY(prep_packed(p.bins, p.vals, p.nb, p.increment, b.tabp, s));
-  if (b.tab64) GGP_TRY(prep_table(p.bins, p.vals, p.nb, 1, 0, b.tab64, b.meta64, s));
+  {
+    StageScope stage("tables", s);
+    GGP_TRY(fill_u32(b.ctr, 0u, 4, s));
+    GGP_TRY(prep_table(p.bins, p.vals, p.nb, sizeof(T) == 8, p.increment, b.tab, b.meta, s));
+    if (b.tabp) GGP_TRY(prep_packed(p.bins, p.vals, p.nb, p.increment, b.tabp, s));
+    if (b.tab64) GGP_TRY(prep_table(p.bins, p.vals, p.nb, 1, 0, b.tab64, b.meta64, s));
+  }
   RefineArgs<T> a{};
   a.mode = mode;
   a.perm = p.perm;
   if (mode == kGenerate) {
     // graph-order buffer: seeds (given or from the dense block) then sentinels
     if (p.with_seed) {
+      StageScope stage("seed_block", s);
       hipLaunchKernelGGL(seed_fill_kernel<T>, dim3(grid_for(p.n, 256)), dim3(256), 0, s, b.vals, (const void*)nullptr,
                          0, (int64_t)0, p.n);
       GGP_LAUNCH_CHECK();
@@ -665,11 +675,14 @@ int refine_flow(const RefineProblem& p, int mode, const RefineIO& io, void* ws, 
       pw.pts = p.sched_pts;
       pw.nbrs = p.sched_nbrs;
       a.sched = p.sched;
+      StageScope stage("refine", s);
       GGP_TRY(run_kernel<T>(a, pw, b, s));
     } else {
+      StageScope stage("refine", s);
       GGP_TRY(run_kernel<T>(a, p, b, s));
     }
     if (p.with_seed) {
+      StageScope stage("nan_fill", s);
       hipLaunchKernelGGL(nan_fill_kernel, dim3(grid_for(std::min<int64_t>(p.n, 1 << 20), 256)), dim3(256), 0, s,
                          io.out, io.out_f64, p.n, b.ctr + 1);
       GGP_LAUNCH_CHECK();
@@ -684,8 +697,12 @@ int refine_flow(const RefineProblem& p, int mode, const RefineIO& io, void* ws, 
     a.out = io.out;
     a.out_f64 = io.out_f64;
     a.io_base = p.with_seed ? 0 : n0;
-    GGP_TRY(run_kernel<T>(a, p, b, s));
+    {
+      StageScope stage("refine", s);
+      GGP_TRY(run_kernel<T>(a, p, b, s));
+    }
     if (p.with_seed) {
+      StageScope stage("seed_block", s);
       GGP_TRY(dense_factor(p.pts, p.pts_f64, p.d, n0, p.bins, p.vals, p.nb, b.dense, s));
       hipLaunchKernelGGL(seed_vals_to_f64_kernel<T>, dim3(grid_for(n0, 256)), dim3(256), 0, s, b.vals, n0, b.dvec);
       GGP_LAUNCH_CHECK();
@@ -706,7 +723,10 @@ int refine_flow(const RefineProblem& p, int mode, const RefineIO& io, void* ws, 
   }
   // log-det
   a.vals = nullptr;
-  GGP_TRY(run_kernel<T>(a, p, b, s));
+  {
+    StageScope stage("refine", s);
+    GGP_TRY(run_kernel<T>(a, p, b, s));
+  }
   const int64_t np = p.n_ref == 0 ? 0 : (p.k + 1 > 40 || p.d > 8 || (p.d > 3 && p.k + 1 > 16)) ? (p.n_ref + 63) / 64 : -1;
   int64_t ntiles = np;
   if (ntiles < 0) {
@@ -716,11 +736,13 @@ int refine_flow(const RefineProblem& p, int mode, const RefineIO& io, void* ws, 
   }
   const double* seed = nullptr;
   if (p.with_seed && n0 > 0) {
+    StageScope stage("seed_block", s);
     GGP_TRY(dense_factor(p.pts, p.pts_f64, p.d, n0, p.bins, p.vals, p.nb, b.dense, s));
     GGP_TRY(dense_logdiag(b.dense, n0, b.dout, s));
     GGP_TRY(dense_logdet_fallback(p.pts, p.pts_f64, p.d, n0, p.bins, p.vals, p.nb, b.dense, b.dout, s));
     seed = b.dout;
   }
+  StageScope stage("logdet_sum", s);
   hipLaunchKernelGGL(logdet_sum_kernel, dim3(1), dim3(1024), 0, s, b.partials, ntiles, seed, (double*)io.out);
   GGP_LAUNCH_CHECK();
   return kOk;

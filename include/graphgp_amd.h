@@ -243,6 +243,18 @@ int ggp_refine_counters(const void* workspace, uint32_t* host_counters, void* st
  * switch off. */
 int ggp_profile_events(void* start_event, void* stop_event);
 
+/* ---- stage timing (no reference counterpart; for benchmarks): ggp_stage_profile(1) switches
+ * it on for the calling host thread and clears its sums; from then on the entry points record
+ * a pair of events around each of their stages on the call's stream ("tree", "knn",
+ * "depths", "order" in ggp_build_graph; "schedule"; "tables", "seed_block", "refine" and, in
+ * GGP_PREC_FP32_INC generate, "weights", "fp64_list", "dataflow"; "nan_fill",
+ * "logdet_sum").  ggp_stage_times waits for the recorded events and returns the number of
+ * distinct stages seen, writing up to `capacity` of them: name (a static string), total
+ * milliseconds and number of occurrences, in order of first appearance.
+ * ggp_stage_profile(0) switches it off. */
+int ggp_stage_profile(int enable);
+int ggp_stage_times(int capacity, const char** names, double* total_ms, int* counts);
+
 /* CU budget (no reference counterpart; for a serving loop that runs small and large
  * requests concurrently): the persistent refinement and adjoint launches, and the kNN
  * query (then a ticketed persistent launch), issued by the calling host thread use at most
