@@ -33,6 +33,7 @@ _SIGS = {
     "ggp_query_neighbors": (_int, [_p, _p, _i64, _int, _int, _p, _p, _i64, _int, _p, _p, _c_size, _p]),
     "ggp_compute_depths_workspace": (_c_size, [_i64, _int]),
     "ggp_compute_depths": (_int, [_p, _i64, _int, _i64, _p, _p, _c_size, _p]),
+    "ggp_compute_depths_method": (_int, [_p, _i64, _int, _i64, _p, _int, _p, _c_size, _p]),
     "ggp_compute_depths_status": (_int, [_p, _u32p, _p]),
     "ggp_order_by_depth_workspace": (_c_size, [_i64]),
     "ggp_order_by_depth": (_int, [_p, _i64, _int, _int, _p, _p, _i64, _int, _p, ctypes.c_int32, _p, _p, _p, _p,

@@ -191,7 +191,12 @@ size_t ggp_compute_depths_workspace(int64_t n_ref, int k) { return compute_depth
 
 int ggp_compute_depths(const int32_t* neighbors, int64_t n_ref, int k, int64_t n0, int32_t* depths, void* workspace,
                        size_t workspace_bytes, void* stream) {
-  return compute_depths(neighbors, n_ref, k, n0, depths, workspace, workspace_bytes, S(stream));
+  return compute_depths(neighbors, n_ref, k, n0, depths, kDepthsFlow, workspace, workspace_bytes, S(stream));
+}
+
+int ggp_compute_depths_method(const int32_t* neighbors, int64_t n_ref, int k, int64_t n0, int32_t* depths,
+                              int method, void* workspace, size_t workspace_bytes, void* stream) {
+  return compute_depths(neighbors, n_ref, k, n0, depths, method, workspace, workspace_bytes, S(stream));
 }
 
 int ggp_compute_depths_status(void* workspace, uint32_t* host_status, void* stream) {
@@ -244,7 +249,7 @@ int ggp_build_graph(const void* points, int64_t n, int d, int dtype, int64_t n0,
   GGP_TRY(query_neighbors(g.tpts, g.split, n, d, dtype, nullptr, nullptr, n0, n - n0, k, g.nbrs, g.sub, g.sub_bytes,
                           s));
   stage.emplace("depths", s);
-  GGP_TRY(compute_depths(g.nbrs, n - n0, k, n0, g.depths, g.sub, g.sub_bytes, s));
+  GGP_TRY(compute_depths(g.nbrs, n - n0, k, n0, g.depths, kDepthsFlow, g.sub, g.sub_bytes, s));
   uint32_t status = 0;
   GGP_TRY(ggp_compute_depths_status(g.sub, &status, stream));
   if (status != 0) {

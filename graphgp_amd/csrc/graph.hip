@@ -366,6 +366,119 @@ __global__ void check_graph_kernel(const int32_t* __restrict__ nbrs, int64_t n_r
 
 namespace {
 
+// ---- depths as a dataflow.  In a graph whose neighbours all precede their point (the
+// order the preceding-neighbour query produces), depth_i = 1 + max over i's neighbours'
+// depths, seeds 0 -- the reference's fixed point (graph.py:101-120), computed in one pass
+// in index order instead of by rounds.  Persistent workgroups take tiles of 64 points from
+// a ticket in index order; four lanes per point gather its neighbours' depths, waiting
+// while one is still the sentinel -1 (a cached read first, the coherent re-read only while
+// it is the sentinel), then publish 1 + their maximum.  Every dependency is on a lower
+// index, so on a lower ticket held by a running workgroup: the lowest unfinished point can
+// always finish.  A row with a neighbour that does not precede its point is malformed: it
+// waits for nothing, publishes 0 and sets the status words (ctr[0], ctr[1]).
+__global__ void depth_flow_init_kernel(int32_t* depths, int64_t n, int64_t n0) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) depths[i] = i < n0 ? 0 : -1;
+}
+
+template <int RM>
+__global__ __launch_bounds__(256) void depth_flow_kernel(const int32_t* __restrict__ nbrs, int64_t n_ref, int k,
+                                                         int64_t n0, int32_t* depths, uint32_t* ctr, int64_t ntiles,
+                                                         const uint32_t* yield, uint32_t keep) {
+  constexpr int P = 4, GPT = 256 / P;
+  __shared__ uint32_t s_tile;
+  const int lane = threadIdx.x & (P - 1);
+  const int gbase = (threadIdx.x & 63) & ~(P - 1);
+  for (;;) {
+    if (threadIdx.x == 0) s_tile = yield_now(yield, keep) ? 0xffffffffu : atomicAdd(&ctr[5], 1u);
+    __syncthreads();
+    const int64_t tile = s_tile;
+    __syncthreads();
+    if (tile >= ntiles) break;
+    const int64_t r = tile * GPT + threadIdx.x / P;
+    const bool valid = r < n_ref;
+    const int64_t i = n0 + r;
+    int32_t nb[RM];
+    uint32_t pend = 0;
+    bool row_bad = false;
+#pragma unroll
+    for (int q = 0; q < RM; ++q) {
+      const int j = P * q + lane;
+      nb[q] = 0;
+      if (valid && j < k) {
+        nb[q] = nbrs[r * k + j];
+        if (nb[q] < 0 || (int64_t)nb[q] >= i) row_bad = true;
+        else if (nb[q] >= n0) pend |= 1u << q;
+      }
+    }
+    const bool bad = ((__ballot(row_bad) >> gbase) & 0xFull) != 0;  // the point's four lanes
+    if (bad) pend = 0;
+    int32_t mx = 0;
+    bool done = !valid;
+    for (uint32_t it = 0;; ++it) {
+      if (!done) {
+#pragma unroll
+        for (int q = 0; q < RM; ++q) {
+          if (pend & (1u << q)) {
+            int32_t v = depths[nb[q]];
+            if (v < 0) v = __hip_atomic_load(&depths[nb[q]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (v >= 0) {
+              mx = v > mx ? v : mx;
+              pend &= ~(1u << q);
+            }
+          }
+        }
+        // a point's lanes finish together, so its lanes are active for the shuffles
+        if (((__ballot(pend == 0) >> gbase) & 0xFull) == 0xFull) {
+          int32_t m = mx;
+          const int32_t m1 = __shfl_xor(m, 1, P);
+          m = m1 > m ? m1 : m;
+          const int32_t m2 = __shfl_xor(m, 2, P);
+          m = m2 > m ? m2 : m;
+          if (lane == 0) {
+            __hip_atomic_store(&depths[i], bad ? 0 : m + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (bad) {
+              atomicOr(&ctr[0], 1u);
+              atomicOr(&ctr[1], 1u);
+            }
+          }
+          done = true;
+        }
+      }
+      if (__all(done)) break;
+      if (it > (1u << 22)) {  // not reached for a graph whose rows all passed the check above
+        if (!done && lane == 0) {
+          atomicOr(&ctr[1], 1u);
+          __hip_atomic_store(&depths[i], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+  }
+}
+
+// The dataflow launch (k <= 64); the workspace needs only the counters.
+int depths_by_flow(const int32_t* nbrs, int64_t n_ref, int k, int64_t n0, int32_t* depths, uint32_t* ctr,
+                   hipStream_t s) {
+  const void* kern = k <= 8    ? (const void*)depth_flow_kernel<2>
+                     : k <= 16 ? (const void*)depth_flow_kernel<4>
+                     : k <= 32 ? (const void*)depth_flow_kernel<8>
+                               : (const void*)depth_flow_kernel<16>;
+  const int64_t n = n0 + n_ref;
+  hipLaunchKernelGGL(depth_flow_init_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, depths, n, n0);
+  GGP_LAUNCH_CHECK();
+  const int64_t ntiles = (n_ref + 63) / 64;
+  const int resident = device_cus() * std::max(1, blocks_per_cu(kern, 0));
+  const int grid = (int)std::min<int64_t>(ntiles, persistent_grid(kern, resident, 0));
+  const YieldCfg& y = yield_cfg();
+  const uint32_t keep = yield_keep(grid);
+  void* args[] = {(void*)&nbrs, (void*)&n_ref, (void*)&k, (void*)&n0, (void*)&depths, (void*)&ctr, (void*)&ntiles,
+                  (void*)&y.dev, (void*)&keep};
+  GGP_HIP(hipLaunchKernel(kern, dim3(grid), dim3(256), args, 0, s));
+  return kOk;
+}
+
 struct KahnWs {
   uint32_t* ctr;
   uint32_t *remaining, *cnt, *queue, *child;
@@ -409,11 +522,15 @@ size_t compute_depths_workspace(int64_t n_ref, int k) {
   return w.used + 256;
 }
 
-// Kahn rounds when the workspace holds the transpose (compute_depths_workspace), else
-// sweeps (the workspace then needs only its first 32 bytes).  Synchronises with the host:
-// once after validation and once per 32 rounds.
-int compute_depths(const int32_t* nbrs, int64_t n_ref, int k, int64_t n0, int32_t* depths, void* ws,
+// Depths by one of three methods: the dataflow (one launch, no host synchronisation; k <=
+// 64, larger k falls back to one of the others), Kahn rounds (needs the transpose in the
+// workspace, compute_depths_workspace; synchronises once after validation and once per 32
+// rounds) or sweeps (the workspace needs only its first 32 bytes; one synchronisation per
+// sweep).  All give the same depths; a malformed graph sets the status word.
+int compute_depths(const int32_t* nbrs, int64_t n_ref, int k, int64_t n0, int32_t* depths, int method, void* ws,
                    size_t ws_bytes, hipStream_t s) {
+  GGP_CHECK_ARG(method == kDepthsFlow || method == kDepthsRounds || method == kDepthsSweeps,
+                "compute_depths: unknown method");
   GGP_CHECK_ARG(n0 >= 0 && n_ref >= 0 && k >= 1, "compute_depths: bad sizes");
   GGP_CHECK_ARG(n_ref < (int64_t(1) << 31), "compute_depths: at most 2^31 refined points");
   Workspace w(ws, ws_bytes);
@@ -425,10 +542,16 @@ int compute_depths(const int32_t* nbrs, int64_t n_ref, int k, int64_t n0, int32_
   }
   const int64_t n = n0 + n_ref;
   GGP_TRY(fill_u32(b.ctr, 0u, kCtrWords, s));
+  if (method == kDepthsFlow && k > 64) method = kahn ? kDepthsRounds : kDepthsSweeps;
+  if (method == kDepthsRounds && !kahn) {
+    set_error("compute_depths: workspace too small for Kahn rounds");
+    return kWorkspaceTooSmall;
+  }
+  if (method == kDepthsFlow) return n_ref == 0 ? kOk : depths_by_flow(nbrs, n_ref, k, n0, depths, b.ctr, s);
   hipLaunchKernelGGL(depth_init_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, depths, n);
   GGP_LAUNCH_CHECK();
   if (n_ref == 0) return kOk;
-  if (!kahn) return depths_by_sweeps(nbrs, n_ref, k, n0, depths, b.ctr, s);
+  if (method == kDepthsSweeps) return depths_by_sweeps(nbrs, n_ref, k, n0, depths, b.ctr, s);
 
   GGP_TRY(fill_u32(b.cnt, 0u, n_ref, s));
   hipLaunchKernelGGL(kahn_init_kernel, dim3(budget_grid((const void*)kahn_init_kernel, grid_capped(n_ref, 256))),

@@ -20,7 +20,8 @@ int query_neighbors(const void* pts, const int32_t* split, int64_t n, int d, int
                     hipStream_t s);
 
 size_t compute_depths_workspace(int64_t n_ref, int k);
-int compute_depths(const int32_t* nbrs, int64_t n_ref, int k, int64_t n0, int32_t* depths, void* ws,
+enum DepthsMethod : int { kDepthsFlow = 0, kDepthsRounds = 1, kDepthsSweeps = 2 };
+int compute_depths(const int32_t* nbrs, int64_t n_ref, int k, int64_t n0, int32_t* depths, int method, void* ws,
                    size_t ws_bytes, hipStream_t s);
 size_t order_by_depth_workspace(int64_t n);
 int order_by_depth(const void* pts, int64_t n, int d, int dtype, const int32_t* idx, const int32_t* nbrs,

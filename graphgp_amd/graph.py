@@ -137,22 +137,26 @@ def build_graph(points, *, n0: int, k: int, cuda: bool = False, return_depths: b
     return (g, depths) if return_depths else g
 
 
-def compute_depths(neighbors, *, n0: int, cuda: bool = False, method: str = "rounds"):
+DEPTH_METHODS = {"flow": 0, "rounds": 1, "sweeps": 2}
+
+
+def compute_depths(neighbors, *, n0: int, cuda: bool = False, method: str = "flow"):
     """Longest-path depth of every point (graph.py:101-120); seeds have depth 0.
 
-    ``method="rounds"`` (default) runs Kahn rounds over the transposed graph;
-    ``"sweeps"`` runs in-place sweeps without the transpose (for graphs whose transpose
-    would not fit in device memory).  Both give the reference's fixed point.
+    ``method="flow"`` (default) computes the depths as a dataflow in index order, in one
+    launch; ``"rounds"`` runs Kahn rounds over the transposed graph; ``"sweeps"`` runs
+    in-place sweeps without the transpose (for graphs whose transpose would not fit in
+    device memory).  All give the reference's fixed point.
     """
-    if method not in ("rounds", "sweeps"):
-        raise ValueError(f"method must be 'rounds' or 'sweeps', got {method!r}")
+    if method not in DEPTH_METHODS:
+        raise ValueError(f"method must be one of {tuple(DEPTH_METHODS)}, got {method!r}")
     nb = as_index(neighbors)
     n_ref, k = nb.shape
     L = _lib.lib()
     out = torch.empty(n0 + n_ref, dtype=torch.int32, device=nb.device)
-    ws = _lib.workspace(L.ggp_compute_depths_workspace(n_ref, k) if method == "rounds" else 64)
-    _lib.check(L.ggp_compute_depths(_lib.ptr(nb), n_ref, k, n0, _lib.ptr(out), _lib.ptr(ws), ws.numel(),
-                                    _lib.stream()), "compute_depths")
+    ws = _lib.workspace(L.ggp_compute_depths_workspace(n_ref, k) if method == "rounds" or k > 64 else 64)
+    _lib.check(L.ggp_compute_depths_method(_lib.ptr(nb), n_ref, k, n0, _lib.ptr(out), DEPTH_METHODS[method],
+                                           _lib.ptr(ws), ws.numel(), _lib.stream()), "compute_depths")
     status = ctypes.c_uint32(0)
     _lib.check(L.ggp_compute_depths_status(_lib.ptr(ws), ctypes.byref(status), _lib.stream()), "compute_depths")
     if status.value:

@@ -34,24 +34,48 @@ GiB = 1 << 30
 
 
 # ---------------------------------------------------------------- byte accounting
-def graph_bytes(n: int, d: int, k: int, n0: int, point_bytes: int = 4) -> int:
-    """Device bytes of a resident Graph: points, neighbours, indices (the server builds
-    its graphs without a dataflow schedule)."""
-    return n * d * point_bytes + (n - n0) * k * 4 + n * 4
+ALLOC_SLACK = 2 << 20  # the caching allocator rounds each block up to 2 MiB
+
+
+def scheduled(n: int, n0: int) -> bool:
+    """Whether build_graph gives a graph of this size a dataflow schedule."""
+    from .graph import SCHEDULE_MIN_REFINED
+
+    return n - n0 >= SCHEDULE_MIN_REFINED
+
+
+def graph_bytes(n: int, d: int, k: int, n0: int, point_bytes: int = 4, schedule: bool = False) -> int:
+    """Device bytes of a resident Graph: points, neighbours, indices and, with
+    ``schedule`` for a graph large enough to get one, its dataflow schedule (slot order,
+    neighbour slots, points in slot order)."""
+    b = n * d * point_bytes + (n - n0) * k * 4 + n * 4
+    if schedule and scheduled(n, n0):
+        b += (n - n0) * (k + 1) * 4 + n * d * point_bytes
+    return b
 
 
 def request_bytes(n: int, d: int, k: int, n0: int, n_bins: int, precision: str) -> int:
-    """Device bytes a generate / generate_inv call holds while in flight: workspace
-    (tables, graph-order values, seed block, counters) plus input and output vectors."""
+    """Device bytes a generate / generate_inv call allocates while in flight: the
+    library's own workspace for it (tables, graph-order values, seed block, counters,
+    FP64 recomputation lists) and the output vector (the input belongs to the caller)."""
+    from . import _lib
+
     vb = 8 if precision == "fp64" else 4
-    tables = n_bins * 48 + 4096
-    return tables + n * vb + (n0 * n0 + 4096) * 8 + n0 * 24 + (n // 32 + 64) * 8 + 2 * n * 8 + 65536
+    ws = _lib.lib().ggp_refine_workspace(n, d, n0, k, n_bins, _lib.PREC[precision], 1)
+    return ws + n * vb + 2 * ALLOC_SLACK
 
 
-def build_bytes(n: int, d: int, k: int, n0: int) -> int:
-    """Transient device bytes of build_graph beyond the resulting graph (tree, kNN boxes,
-    transposed graph for the depths, ordering buffers)."""
-    return n * d * 4 * 3 + n * 40 + (n - n0) * k * 4 + n * 48
+def build_bytes(n: int, d: int, k: int, n0: int, schedule: bool = False) -> int:
+    """Transient device bytes of build_graph beyond the resulting graph: the library's
+    build workspace (tree, kNN, depths, ordering), the depth vector and, with
+    ``schedule`` for a graph large enough, the schedule's workspace."""
+    from . import _lib
+
+    L = _lib.lib()
+    b = L.ggp_build_graph_workspace(n, d, 0, n0, k) + n * 4 + 2 * ALLOC_SLACK
+    if schedule and scheduled(n, n0):
+        b += L.ggp_graph_schedule_workspace(n, n0) + ALLOC_SLACK
+    return b
 
 
 # ---------------------------------------------------------------- resident graphs
@@ -204,8 +228,8 @@ class Server:
     those that do not fit yet.
     """
 
-    def __init__(self, hbm_budget_bytes: int = 200 * GiB, max_inflight: int = 64, small_n: int = 1_000_000,
-                 large_cu_fraction: float = 0.35, build_fn: Optional[Callable] = None):
+    def __init__(self, hbm_budget_bytes: int = 200 * 10**9, max_inflight: int = 64, small_n: int = 1_000_000,
+                 large_cu_fraction: float = 0.35, build_fn: Optional[Callable] = None, schedule: bool = False):
         import concurrent.futures as cf
 
         import torch
@@ -218,8 +242,13 @@ class Server:
         self.adm = Admission(hbm_budget_bytes, self.cache)
         self.small_n = small_n
         self.max_inflight = max_inflight
-        # no dataflow schedules: many resident graphs, and builds dominate the mix
-        self.build_fn = build_fn or (lambda pts, n0, k: gp.build_graph(pts, n0=n0, k=k, schedule=False))
+        self.build_depth = 2  # builds per class handed to its builder: the running one and the next
+        # ``schedule``: build large graphs (scheduled(n, n0)) with their dataflow schedule,
+        # whose bytes graph_bytes and build_bytes then count.  Off by default: on the
+        # synthetic mix, where builds dominate and a graph serves about three requests,
+        # computing schedules lowers the batch capacity more than they speed up generate.
+        self.schedule = bool(schedule)
+        self.build_fn = build_fn or (lambda pts, n0, k: gp.build_graph(pts, n0=n0, k=k, schedule=self.schedule))
         small_work, small_build, large = torch.cuda.Stream(), torch.cuda.Stream(), torch.cuda.Stream()
         self.work = [small_work, large]  # [small, large]
         self.build_streams = [small_build, large]
@@ -336,13 +365,16 @@ class Server:
                 del building[key]
                 r0 = parked[0]
                 self.adm.release(reserved)
-                self.cache.put(key, g, graph_bytes(r0.n, r0.d, r0.k, r0.n0))
+                self.cache.put(key, g, graph_bytes(r0.n, r0.d, r0.k, r0.n0, schedule=self.schedule))
                 for req in parked:
                     launch(req, g, after=ev)
                 progressed = True
             return progressed
 
         small = self.small_n
+
+        def builds_of(cls):
+            return sum(1 for b in building.values() if self._cls(b[1][0]) == cls)
 
         def small_active():
             return (any(r.n <= small for r in waiting) or any(r.n <= small for r, _, _ in inflight)
@@ -368,10 +400,14 @@ class Server:
                         launch(req, self.cache.get(key))
                         progressed = True
                         continue
-                else:
+                elif builds_of(self._cls(req)) < self.build_depth:
+                    # a build is handed to its class's builder thread (one build at a time)
+                    # only while fewer than build_depth are there, and reserves its bytes
+                    # only then: queued builds would hold memory that dispatchable work needs
                     if req.points is None:
                         raise ValueError(f"request {req.rid}: graph {key!r} is not resident and no points given")
-                    extra = graph_bytes(req.n, req.d, req.k, req.n0) + build_bytes(req.n, req.d, req.k, req.n0)
+                    extra = (graph_bytes(req.n, req.d, req.k, req.n0, schedule=self.schedule)
+                             + build_bytes(req.n, req.d, req.k, req.n0, schedule=self.schedule))
                     if self.adm.try_admit(need + extra):  # extra is released when the build ends
                         req.built_graph = True
                         building[key] = (self.builders[self._cls(req)].submit(self._build, req), [req], extra)
@@ -490,21 +526,13 @@ def materialize(specs, device="cuda"):
 
 
 def serve_benchmark(n_requests: int = 2000, seed: int = 5, load: float = 0.9,
-                    large_cu_fraction: float = 0.35) -> Dict[str, Any]:
-    """The serving measurement: the synthetic mix served all at once (batch mode, the
-    server's capacity), then with Poisson arrivals at ``load`` x that request rate, both
-    with the same server policy; fresh servers (no resident graphs) for every run.  Also
-    the batch throughput with no CU reservation for small requests, so that the cost of
-    the reservation shows."""
+                    large_cu_fraction: float = 0.35, schedule: bool = False) -> Dict[str, Any]:
+    """The serving measurement: the synthetic mix served all at once with no CU
+    reservation (batch mode: the device's capacity on this mix), then with Poisson
+    arrivals at ``load`` x that request rate under the serving policy (small requests
+    reserve CUs while active); fresh servers (no resident graphs) for both runs.
+    ``serve_vs_batch`` is the served throughput over the batch capacity."""
     import torch
-
-    def batch_run(frac):
-        reqs = materialize(synthetic_mix(n_requests, rate_per_s=1e9, seed=seed))
-        torch.cuda.synchronize()
-        srv = Server(large_cu_fraction=frac)
-        out = srv.run(reqs, realtime=False)
-        srv.close()
-        return out
 
     def fresh():  # every run starts from an empty device cache
         import gc
@@ -513,23 +541,28 @@ def serve_benchmark(n_requests: int = 2000, seed: int = 5, load: float = 0.9,
         torch.cuda.synchronize()
         torch.cuda.empty_cache()
 
-    full = batch_run(1.0)
-    fresh()
-    batch = batch_run(large_cu_fraction)
+    reqs = materialize(synthetic_mix(n_requests, rate_per_s=1e9, seed=seed))
+    torch.cuda.synchronize()
+    srv = Server(large_cu_fraction=1.0, schedule=schedule)
+    batch = srv.run(reqs, realtime=False)
+    srv.close()
+    del srv, reqs
     fresh()
     rate = load * batch["requests"] / batch["wall_s"]
     reqs = materialize(synthetic_mix(n_requests, rate_per_s=rate, seed=seed))
     torch.cuda.synchronize()
-    srv = Server(large_cu_fraction=large_cu_fraction)
+    srv = Server(large_cu_fraction=large_cu_fraction, schedule=schedule)
     serve = srv.run(reqs, realtime=True)
     srv.close()
+    del srv, reqs
+    fresh()
     return {
         "requests": n_requests,
         "arrival_rate_per_s": round(rate, 2),
         "load": load,
         "large_cu_fraction": large_cu_fraction,
+        "schedules": schedule,
         "batch_points_per_s": round(batch["points_per_s"], 1),
-        "batch_points_per_s_no_reservation": round(full["points_per_s"], 1),
         "serve_points_per_s": round(serve["points_per_s"], 1),
         "serve_vs_batch": round(serve["points_per_s"] / batch["points_per_s"], 4),
         "p50_ms": round(serve["p50_ms"], 3),
@@ -549,5 +582,7 @@ if __name__ == "__main__":
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--seed", type=int, default=5)
     ap.add_argument("--load", type=float, default=0.9)
+    ap.add_argument("--large-cu-fraction", type=float, default=0.35)
+    ap.add_argument("--schedule", action="store_true", help="build large graphs with their dataflow schedule")
     a = ap.parse_args()
-    print(json.dumps(serve_benchmark(a.requests, a.seed, a.load)))
+    print(json.dumps(serve_benchmark(a.requests, a.seed, a.load, a.large_cu_fraction, a.schedule)), flush=True)

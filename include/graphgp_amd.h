@@ -61,12 +61,19 @@ int ggp_query_neighbors(const void* points, const int32_t* split_dims, int64_t n
 /* ---- depths: replaces graphgp_cuda.compute_depths_parallel(neighbors, n0=),
  * graphgp/graph.py:106.  depths (n0 + n_ref).  A malformed graph (a neighbour that does
  * not precede its point) is reported by ggp_compute_depths_status (syncs): status 1.
- * With the workspace of ggp_compute_depths_workspace (holds the transposed graph) the
- * depths come from Kahn rounds; a workspace of only 64 bytes selects in-place sweeps
- * (slower, no transpose) for graphs whose transpose would not fit in device memory. */
+ * ggp_compute_depths computes them as a dataflow in index order (one launch, no host
+ * synchronisation, a workspace of 64 bytes; for k > 64 it falls back to Kahn rounds when
+ * the workspace holds ggp_compute_depths_workspace bytes, else to sweeps).
+ * ggp_compute_depths_method (no reference counterpart) selects the method:
+ * GGP_DEPTHS_FLOW, GGP_DEPTHS_ROUNDS (Kahn rounds over the transposed graph, workspace of
+ * ggp_compute_depths_workspace) or GGP_DEPTHS_SWEEPS (in-place sweeps, 64 bytes; for
+ * graphs whose transpose would not fit).  Every method gives the same depths. */
+enum { GGP_DEPTHS_FLOW = 0, GGP_DEPTHS_ROUNDS = 1, GGP_DEPTHS_SWEEPS = 2 };
 size_t ggp_compute_depths_workspace(int64_t n_ref, int k);
 int ggp_compute_depths(const int32_t* neighbors, int64_t n_ref, int k, int64_t n0, int32_t* depths, void* workspace,
                        size_t workspace_bytes, void* stream);
+int ggp_compute_depths_method(const int32_t* neighbors, int64_t n_ref, int k, int64_t n0, int32_t* depths,
+                              int method, void* workspace, size_t workspace_bytes, void* stream);
 int ggp_compute_depths_status(void* workspace, uint32_t* host_status, void* stream);
 
 /* ---- depth order: replaces graphgp_cuda.order_by_depth(points, indices, neighbors,

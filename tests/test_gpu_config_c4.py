@@ -38,7 +38,7 @@ def test_c4_stream_alone_equals_mix():
     for i in sample:
         r = reqs[i]
         if r.graph_key not in graphs:
-            graphs[r.graph_key] = gp.build_graph(r.points, n0=r.n0, k=r.k, schedule=False)
+            graphs[r.graph_key] = gp.build_graph(r.points, n0=r.n0, k=r.k, schedule=False)  # as the server builds
         g = graphs[r.graph_key]
         fn = gp.generate if r.op == "generate" else gp.generate_inv
         alone = fn(g, r.covariance, r.data, precision=r.precision)

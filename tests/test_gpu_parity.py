@@ -132,14 +132,14 @@ def test_query_n0_less_than_k_raises():
 
 
 # ---------------------------------------------------------------- graph
-@pytest.mark.parametrize("method", ["rounds", "sweeps"])
+@pytest.mark.parametrize("method", ["flow", "rounds", "sweeps"])
 def test_compute_depths_kat(method):
     # tests/test_graph.py:40-52
     nb = np.array([[0, 1, 2], [0, 1, 2], [1, 3, 4], [2, 4, 5]])
     assert np.array_equal(cpu(gp.compute_depths(nb, n0=3, method=method)), [0, 0, 0, 1, 1, 2, 3])
 
 
-@pytest.mark.parametrize("method", ["rounds", "sweeps"])
+@pytest.mark.parametrize("method", ["flow", "rounds", "sweeps"])
 def test_compute_depths_random_vs_oracle(method):
     # random graphs in topological order with repeated neighbours, and a 3000-long chain
     # (3000 rounds: more than one batch of rounds between host checks)
@@ -155,7 +155,7 @@ def test_compute_depths_random_vs_oracle(method):
         assert np.array_equal(d, O.compute_depths(nb, n0=n0)), (n0, n_ref, k)
 
 
-@pytest.mark.parametrize("method", ["rounds", "sweeps"])
+@pytest.mark.parametrize("method", ["flow", "rounds", "sweeps"])
 def test_compute_depths_hub_and_chain(method):
     # one refined point is a neighbour of every later point (thousands of children), the
     # second column chains the points, the third is random
@@ -179,7 +179,7 @@ def test_order_by_depth_kat():
     assert np.array_equal(cpu(d), [0, 0, 1, 1, 2, 3])
 
 
-@pytest.mark.parametrize("method", ["rounds", "sweeps"])
+@pytest.mark.parametrize("method", ["flow", "rounds", "sweeps"])
 def test_compute_depths_rejects_malformed(method):
     with pytest.raises(ValueError):
         gp.compute_depths(np.array([[0, 4], [1, 2]]), n0=3, method=method)
