@@ -558,6 +558,8 @@ int run_two_phase(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hipSt
   RefineProblem pw = p;  // run_kernel_inner sets the kernel's inputs from its problem
   pw.pts = w_pts;
   pw.nbrs = w_nbrs;
+  // the weights phases store sigma xi per slot (xi through the schedule and the permutation)
+  a.sched = p.sched;
   a.mode = kWeights;
   GGP_TRY(run_kernel_inner<T>(a, pw, b, s));
   // the profiling events bracket the dominant kernel: the weights phase
@@ -586,6 +588,11 @@ int run_two_phase(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hipSt
     d.w64 = b.w64;
     d.sig64 = b.sig64;
     d.list_in = b.fix_list;
+    d.sched = p.sched;
+    d.perm = p.perm;
+    d.xi = a.xi;
+    d.xi_f64 = a.xi_f64;
+    d.io_base = a.io_base;
     const size_t lds64 = (size_t)(p.nb - 1) * sizeof(TabE<double>) + (size_t)p.nb * sizeof(double);
     d.fix_lds_offset = (int)((lds64 + 15) & ~size_t(15));
     GGP_TRY(launch_refine<double>(d, p.k + 1, s, lds64, device_cus() * 4));
@@ -596,11 +603,13 @@ int run_two_phase(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hipSt
   a.mode = kGenerate;
   a.pts = p.pts;
   a.nbrs = p.nbrs;
-  a.sched = p.sched;
   a.nbv = p.sched_nbrs;
-  constexpr int P = 4, RM = 10;  // k <= 40 for the register configurations
+  constexpr int P = 4;  // rows per lane: the smallest of 2, 4, 8, 10 covering k (k <= 40 here)
   a.ntiles = (p.n_ref + 256 / P - 1) / (256 / P);
-  auto kern = apply_kernel<T, P, RM>;
+  auto kern = p.k <= 2 * P ? apply_kernel<T, P, 2>
+              : p.k <= 4 * P ? apply_kernel<T, P, 4>
+              : p.k <= 8 * P ? apply_kernel<T, P, 8>
+                             : apply_kernel<T, P, 10>;
   const int grid = (int)std::min<int64_t>(a.ntiles, persistent_grid((const void*)kern, cus * 8, 0));
   a.yield = yield_cfg().dev;
   a.groups = std::min(grid, kMaxGroups);

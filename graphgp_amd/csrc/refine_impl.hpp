@@ -115,10 +115,10 @@ struct RefineArgs {
   double fix_tau;
   uint32_t* fix_count;
   int fix_lds_offset;
-  // two-phase generate: weights w (n_ref x k) and sigma of every point (mean = w . u, with
-  // u the neighbour values, in increments of neighbour 0 in the increment formulation);
-  // the points needing the FP64 recomputation, their list slot (-1: none), FP64 weights
-  // and sigma
+  // two-phase generate: weights w (n_ref x k) and sigma xi of every point (mean = w . u,
+  // with u the neighbour values, in increments of neighbour 0 in the increment
+  // formulation); the points needing the FP64 recomputation, their list slot (-1: none),
+  // FP64 weights and sigma xi
   T* wts;
   T* sig;
   int32_t* fixslot;
@@ -484,7 +484,9 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
   const int lane = threadIdx.x & (P - 1);
   const int grp = threadIdx.x / P;
   const int k_arg = a.k;
-  const bool incr = a.increment != 0;
+  // the weights phases run in one formulation each (FP32: increments; FP64 list: plain),
+  // fixed at compile time there
+  const bool incr = MODE == kWeights ? sizeof(T) == 4 : a.increment != 0;
   const bool fixing = a.tab64 != nullptr;
   const T tau = (T)a.fix_tau;
   // FP64 recomputation: the first kFixWaves wavefronts have an LDS slab each
@@ -518,6 +520,13 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
       pt = valid ? (int64_t)a.list_in[slot_in] : 0;
     }
     const int64_t fine = a.n0 + pt;
+    // weights phases: xi of the point, issued now and used at the end (sigma xi is stored);
+    // pt is the point's slot, whose graph position comes from the schedule
+    T xw = T(0);
+    if (mode == kWeights && valid) {
+      const int64_t g = a.n0 + (a.sched != nullptr ? (int64_t)a.sched[pt] : pt);
+      xw = load_as<T>(a.xi, a.xi_f64, (a.perm != nullptr ? (int64_t)a.perm[g] : g) - a.io_base);
+    }
 
     // ---- gather this lane's rows: neighbours first, fine point at row k
     T c[R][D];
@@ -666,7 +675,7 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
 #pragma unroll
         for (int r = 0; r < R; ++r)
           if (P * r + lane < k) a.w64[slot_in * k + P * r + lane] = (double)wv[r];
-        if (owner) a.sig64[slot_in] = (double)sqrt(skk);
+        if (owner) a.sig64[slot_in] = (double)sqrt(skk) * (double)xw;
       } else if (valid) {
 #pragma unroll
         for (int r = 0; r < R; ++r)
@@ -674,7 +683,7 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
         if (owner) {
           // a NaN here is not yet an error: a flagged point's FP32 sigma is replaced by its
           // FP64 recomputation, and the published values are checked in the last phase
-          a.sig[pt] = sqrt(skk);
+          a.sig[pt] = sqrt(skk) * xw;
           int32_t slot = -1;
           if (flagged) {
             const uint32_t q = atomicAdd(a.fix_count, 1u);
@@ -844,12 +853,18 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
   }
 }
 
-// Last phase of the two-phase generate: the dataflow over the stored weights, P lanes per point, lane j % P gathers
-// neighbour j.  A point waits until all its neighbours' values are published (sentinel =
-// unpublished; a cached read first, the coherent re-read only while it is still the
-// sentinel), then publishes v = [v0] + w . u + sigma xi (FP64 weights and sum for the
-// recomputed points).  Persistent workgroups take tiles from an atomic ticket in index order,
-// so every dependency is on a lower ticket held by a running workgroup.
+// Last phase of the two-phase generate: the dataflow over the stored weights, P lanes per
+// point, lane j % P gathers neighbour j.  A point waits until all its neighbours' values are
+// published (sentinel = unpublished; a cached read first, the coherent re-read only while it
+// is still the sentinel), then publishes v = [v0] + w . u + sigma xi (FP64 weights and sum
+// for the recomputed points); the weights phases stored sigma xi per slot, so no xi is
+// gathered here.  Persistent workgroups take tiles from an atomic ticket in index order, so
+// every dependency is on a lower ticket held by a running workgroup.
+//
+// The loads of a row stay behind that row's own test: a variant that issued every row's
+// loads at once (clamped addresses, one wait per batch, coherent re-reads of all pending rows
+// per poll round) took 4.2-4.7 ms against 3.6 ms for this form at 16.8M points, k = 30; the
+// phase is bound by memory traffic, and the extra coherent reads only added to it.
 template <typename T, int P, int RM>
 __global__ __launch_bounds__(256) void apply_kernel(RefineArgs<T> a) {
   constexpr int GPT = 256 / P;
@@ -878,8 +893,8 @@ __global__ __launch_bounds__(256) void apply_kernel(RefineArgs<T> a) {
     const int64_t q = tile * GPT + grp;  // slot: the refined point's index, or its schedule slot
     const bool valid = q < a.n_ref;
     const int64_t pt = (a.sched != nullptr && valid) ? (int64_t)a.sched[q] : q;
-    const int64_t fine = a.n0 + pt;  // graph position: xi and output through perm
-    const int64_t store = a.n0 + q;  // value slot; weights, sigma and fixslot are slot-indexed
+    const int64_t fine = a.n0 + pt;  // graph position: output through perm
+    const int64_t store = a.n0 + q;  // value slot; weights, sigma xi and fixslot are slot-indexed
     const int32_t slot = valid ? a.fixslot[q] : -1;
     int32_t nb[RM];
     double wd[RM];
@@ -895,13 +910,12 @@ __global__ __launch_bounds__(256) void apply_kernel(RefineArgs<T> a) {
         pend |= 1u << r;
       }
     }
-    T xi = T(0), sg = T(0);
+    T sx = T(0);
     int64_t dst = fine;
     const bool owner = lane == 0;
     if (valid && owner) {
       dst = a.perm ? (int64_t)a.perm[fine] : fine;
-      xi = load_as<T>(a.xi, a.xi_f64, dst - a.io_base);
-      sg = slot >= 0 ? (T)0 : a.sig[q];
+      sx = slot >= 0 ? (T)0 : a.sig[q];
     }
     T u[RM];
 #pragma unroll
@@ -929,7 +943,7 @@ __global__ __launch_bounds__(256) void apply_kernel(RefineArgs<T> a) {
             for (int r = 0; r < RM; ++r) part += wd[r] * (double)u[r];
             // a group's lanes finish together, so the group sum's DPP sources are active
             part = group_sum<P>(part);
-            v = (T)(part + a.sig64[slot] * (double)xi);
+            v = (T)(part + a.sig64[slot]);
           } else {
             // neighbour 0 is lane 0's first slot
             const T v0 = incr ? bcast<P, 0>(u[0]) : T(0);
@@ -941,7 +955,7 @@ __global__ __launch_bounds__(256) void apply_kernel(RefineArgs<T> a) {
               part += (T)wd[r] * uj;
             }
             part = group_sum<P>(part);
-            v = v0 + part + sg * xi;
+            v = v0 + part + sx;
           }
           if (owner) {
             if (v != v) {
