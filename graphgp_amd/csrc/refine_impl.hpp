@@ -4,6 +4,7 @@
 #include <type_traits>
 
 #include "common.hpp"
+#include "refine_dpp.hpp"
 
 namespace ggp {
 
@@ -150,6 +151,15 @@ struct RefineArgs {
 };
 
 // ------------------------------------------------------------------ helpers
+// FP32, four lanes per point: the trailing update of column COL in column MM (slot MS,
+// lane ML) for the lane's rows in slots MS..R-1, m[r][MM] -= m[r][COL] * (lane ML's
+// m[MS][COL]), as one statement of DPP-fused FMAs.
+template <int ML, int MS, int P, int COL, int MM, int S, int... Is>
+__device__ __forceinline__ void update_rows_dpp(float (&m)[S], std::integer_sequence<int, Is...>) {
+  FmacDpp<ML, sizeof...(Is)>::run(m[P * MS * (MS + 1) / 2 + COL], m[P * (MS + Is) * (MS + Is + 1) / 2 + MM]...,
+                                  m[P * (MS + Is) * (MS + Is + 1) / 2 + COL]...);
+}
+
 template <int B, int E, typename F>
 __device__ __forceinline__ void static_for(F&& f) {
   if constexpr (B < E) {
@@ -595,9 +605,13 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
           constexpr int mm = decltype(mc)::value;
           constexpr int ms = mm / P, ml = mm % P;
           if (mm <= k) {
-            const T b = bcast<P, ml>(m[P * ms * (ms + 1) / 2 + col]);
+            if constexpr (sizeof(T) == 4 && P == 4) {
+              update_rows_dpp<ml, ms, P, col, mm>(m, std::make_integer_sequence<int, R - ms>());
+            } else {
+              const T b = bcast<P, ml>(m[P * ms * (ms + 1) / 2 + col]);
 #pragma unroll
-            for (int r = ms; r < R; ++r) m[P * r * (r + 1) / 2 + mm] -= m[P * r * (r + 1) / 2 + col] * b;
+              for (int r = ms; r < R; ++r) m[P * r * (r + 1) / 2 + mm] -= m[P * r * (r + 1) / 2 + col] * b;
+            }
           }
         });
       }

@@ -46,7 +46,11 @@ def _counters(ws):
 
     buf = (ctypes.c_uint32 * 4)()
     _lib.check(_lib.lib().ggp_refine_counters(_lib.ptr(ws), buf, _lib.stream()), "refine counters")
-    return {"tiles": int(buf[0]), "nan": int(buf[1]), "malformed": int(buf[2]), "fp64_recomputed": int(buf[3])}
+    # error word bits: 1 = a wait gave up (malformed graph), 2 = more FP32-increment points
+    # needed the FP64 recomputation than its list holds (a quarter of the points); the rest
+    # kept their FP32 weights
+    return {"tiles": int(buf[0]), "nan": int(buf[1]), "malformed": int(buf[2] & 1),
+            "fp64_list_overflow": int((buf[2] >> 1) & 1), "fp64_recomputed": int(buf[3])}
 
 
 def generate(graph: Graph, covariance, xi, *, cuda: bool = False, fast_jit: bool = True, precision=None,

@@ -14,7 +14,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+# GGP_ORACLE_LIB selects another build of the same source (the AddressSanitizer build)
+_LIB_PATH = os.environ.get("GGP_ORACLE_LIB") or os.path.join(_HERE, "_build", "liboracle.so")
 _lib = None
 
 _i64 = ctypes.c_int64
