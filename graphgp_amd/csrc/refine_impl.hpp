@@ -508,12 +508,21 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
   const int gbase = (threadIdx.x & 63) & ~(P - 1);
   const uint64_t gmask = (P == 64) ? ~0ull : ((1ull << P) - 1ull);
 
+  // Tickets are taken one tile ahead: thread 0 takes the next tile's ticket at the start of
+  // the current tile, so its round trip overlaps the tile's work (left uninitialised in the
+  // other threads, so that no value meets the atomic's result at the branch's end, which
+  // would make the wavefront wait for it there).  In the dataflow modes a workgroup then
+  // holds its current tile and a higher one; the lowest unfinished tile is still held by a
+  // workgroup whose earlier tiles are finished, so every dependency still completes.
+  uint32_t next_ticket;
+  if (threadIdx.x == 0) next_ticket = yield_now(a.yield, a.keep) ? 0xffffffffu : atomicAdd(a.ticket, 1u);
   for (;;) {
-    if (threadIdx.x == 0) s_tile = yield_now(a.yield, a.keep) ? 0xffffffffu : atomicAdd(a.ticket, 1u);
+    __syncthreads();  // every thread is done with the previous tile (and its ticket read)
+    if (threadIdx.x == 0) s_tile = next_ticket;
     __syncthreads();
     const int64_t tile = s_tile;
-    __syncthreads();
     if (tile >= a.ntiles) break;
+    if (threadIdx.x == 0) next_ticket = yield_now(a.yield, a.keep) ? 0xffffffffu : atomicAdd(a.ticket, 1u);
     // k anew in every tile, opaque to the compiler (an SGPR it may not look through): the
     // unrolled column tests below are then scalar compares where they are used, instead of
     // being hoisted out of the tile loop as one lane mask each, which spill.
@@ -574,10 +583,30 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
         for (int t = 0; t < D; ++t) cj[t] = bcast<P, js>(c[jr][t]);
         T gj = T(0);
         if (incr) gj = bcast<P, js>(g0[jr]);
+        T tv[R];  // the table values of this column's entries
+        if constexpr (PACKED && FAST) {
+          // in three passes, so that the column's LDS reads are in flight together: the
+          // segments, the reads, the interpolations
+          T r2v[R];
+          int ix[R];
+#pragma unroll
+          for (int r = jr; r < R; ++r) {
+            r2v[r] = pair_dist2<T, D>(c[r], cj);
+            ix[r] = packed_index(r2v[r], ka, kb, nseg);
+          }
+          TabP e[R];
+#pragma unroll
+          for (int r = jr; r < R; ++r) e[r] = ptab[ix[r]];
+#pragma unroll
+          for (int r = jr; r < R; ++r) tv[r] = e[r].c + (__builtin_amdgcn_sqrtf(r2v[r]) - e[r].e) * e[r].s;
+        } else {
+#pragma unroll
+          for (int r = jr; r < R; ++r) tv[r] = lookup2(pair_dist2<T, D>(c[r], cj));
+        }
 #pragma unroll
         for (int r = jr; r < R; ++r) {
           const int i = P * r + lane;
-          T v = lookup2(pair_dist2<T, D>(c[r], cj));
+          T v = tv[r];
           if (incr) {
             if (j == 0) v = (i == 0) ? f_zero : -g0[r];
             else v = (i == 0) ? -gj : (g0[r] + gj) - v;
