@@ -22,7 +22,11 @@
 //
 // MI355X mapping: the k slots live in VGPRs (k <= 64, compile-time slot count KMAX with
 // slots >= k pinned at -inf so they never hold the maximum); the final order is a
-// register bitonic network.  Larger k uses slots in a caller workspace.
+// register bitonic network.  Larger k uses slots in a caller workspace.  Each visit issues
+// all of its loads at once (the node's point and split, both children's boxes), so it
+// costs one memory round trip; up to k = 32 the kernel is held to three wavefronts per SIMD
+// (168 VGPRs).  At 16.8M clustered points, k = 30: 263 -> 228 ms against two round trips
+// per visit.
 #include "common.hpp"
 #include "graphgp_amd_internal.hpp"
 
@@ -57,21 +61,8 @@ __device__ inline T dist2(const Query<T, D>& q, const T* __restrict__ p, int d) 
 }
 
 // box of node c: lo at box[c*2d + j], hi at box[c*2d + d + j]; distance^2 from q to the
-// box, accumulated in the order of dist2.
-template <typename T, int D>
-__device__ inline T box_dist2(const Query<T, D>& q, const T* __restrict__ b, int d) {
-  T s = T(0);
-#pragma unroll
-  for (int j = 0; j < D; ++j) {
-    if (j < d) {
-      const T lo = b[j] - q.c[j], hi = q.c[j] - b[d + j];
-      const T g = lo > T(0) ? lo : (hi > T(0) ? hi : T(0));
-      s = j == 0 ? g * g : s + g * g;
-    }
-  }
-  return s;
-}
-
+// box, accumulated in the order of dist2 (the register kernel computes the same sum from
+// boxes it has loaded with the node).
 template <typename T>
 __device__ inline T box_dist2_rt(const T* __restrict__ q, const T* __restrict__ b, int d) {
   T s = T(0);
@@ -138,9 +129,31 @@ __device__ __forceinline__ void knn_query(const T* __restrict__ pts, const int32
   int64_t cur = 0, prev = -1;
   while (cur != -1) {
     const int64_t parent = tree_parent(cur);
-    const T* p = pts + cur * d;
+    const int64_t l = tree_left(cur), r = tree_right(cur);
+    // Every load of this visit at once, so a visit costs one memory round trip: the node's
+    // point and split dimension, and the boxes of both children (which one is near is only
+    // known from the point; a child that does not exist reads the node's own box, unused).
+    // (coordinates j >= d, which the instantiation carries beyond the data's dimension,
+    // read coordinate d - 1 and are unused: a load under a branch would wait at its join)
+    Query<T, D> p;
+#pragma unroll
+    for (int j = 0; j < D; ++j) p.c[j] = pts[cur * d + (j < d ? j : d - 1)];
+    const int dim = split[cur];
+    T bl[2 * D], br[2 * D];
+    {
+      const T* bpl = box + (l < n ? l : cur) * 2 * d;
+      const T* bpr = box + (r < n ? r : cur) * 2 * d;
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        const int jc = j < d ? j : d - 1;
+        bl[j] = bpl[jc];
+        bl[D + j] = bpl[d + jc];
+        br[j] = bpr[jc];
+        br[D + j] = bpr[d + jc];
+      }
+    }
     if (prev == parent) {
-      const T s = dist2<T, D>(q, p, d);
+      const T s = dist2<T, D>(q, p.c, d);
       if (s < r2) {
 #pragma unroll
         for (int j = 0; j < KMAX; ++j)
@@ -152,16 +165,29 @@ __device__ __forceinline__ void knn_query(const T* __restrict__ pts, const int32
           if (sd[j] > r2) { r2 = sd[j]; amax = j; }
       }
     }
-    const int dim = split[cur];
-    const T sdist = q.get(dim) - p[dim];
-    const int64_t l = tree_left(cur), r = tree_right(cur);
-    const int64_t nearc = sdist < T(0) ? l : r;
-    const int64_t farc = sdist < T(0) ? r : l;
+    const T sdist = q.get(dim) - p.get(dim);
+    const bool left_near = sdist < T(0);
+    const int64_t nearc = left_near ? l : r;
+    const int64_t farc = left_near ? r : l;
     const bool far_in = sdist * sdist <= r2;
+    // box distance of the near and far child, from the registers loaded above
+    auto child_box_dist2 = [&](bool near) -> T {
+      const bool left = near == left_near;
+      T s2 = T(0);
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        if (j < d) {
+          const T blo = left ? bl[j] : br[j], bhi = left ? bl[D + j] : br[D + j];
+          const T lo = blo - q.c[j], hi = q.c[j] - bhi;
+          const T g = lo > T(0) ? lo : (hi > T(0) ? hi : T(0));
+          s2 = j == 0 ? g * g : s2 + g * g;
+        }
+      }
+      return s2;
+    };
     int64_t next;
-    if (prev == nearc ||
-        (prev == parent && (nearc >= max_index || box_dist2<T, D>(q, box + nearc * 2 * d, d) > r2)))
-      next = (farc < max_index && far_in && !(box_dist2<T, D>(q, box + farc * 2 * d, d) > r2)) ? farc : parent;
+    if (prev == nearc || (prev == parent && (nearc >= max_index || child_box_dist2(true) > r2)))
+      next = (farc < max_index && far_in && !(child_box_dist2(false) > r2)) ? farc : parent;
     else
       next = prev == parent ? nearc : parent;
     prev = cur;
@@ -201,7 +227,7 @@ __device__ __forceinline__ void knn_query(const T* __restrict__ pts, const int32
 // launch capped below the query count, so that it leaves room for other work on the
 // device): workgroups take blocks of 256 queries from the ticket until none are left.
 template <typename T, int D, int KMAX, bool TICKET>
-__global__ __launch_bounds__(256) void knn_kernel(const T* __restrict__ pts, const int32_t* __restrict__ split,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KMAX <= 32 ? 3 : 1))) void knn_kernel(const T* __restrict__ pts, const int32_t* __restrict__ split,
                                                   int d, const int32_t* __restrict__ qidx,
                                                   const int32_t* __restrict__ midx, int64_t q0, int64_t nq, int k,
                                                   int32_t* __restrict__ out, int64_t n, const T* __restrict__ box,
