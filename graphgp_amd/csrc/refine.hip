@@ -336,7 +336,7 @@ __global__ void seed_out_kernel(const double* __restrict__ sv, const int32_t* __
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n0) return;
   const double v = sv[i];
-  if (v != v) atomicOr(&flags[0], 1u);
+  if (v != v) atomicOr(&flags[0], 2u);  // bit 2: the seed block failed (bit 1: a refined value)
   if (vals) vals[i] = (T)v;
   if (out) {
     const int64_t dst = perm ? (int64_t)perm[i] : i;
@@ -349,6 +349,29 @@ template <typename T>
 __global__ void seed_vals_to_f64_kernel(const T* __restrict__ vals, int64_t n0, double* __restrict__ v) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n0) v[i] = (double)vals[i];
+}
+
+// FP64 fallback of the two-phase generate.  When more points need the FP64 recomputation
+// than its list holds (error bit 2, set by the weights phase), the FP32-increment values
+// are not within their bound, so the whole refinement is recomputed in FP64 (the
+// precision="fp64" kernel) into a value buffer of its own, and its values, rounded, are the
+// output.  Each kernel of the fallback returns at once unless the bit is set.
+__global__ void fallback_fill_kernel(double* __restrict__ vals64, const double* __restrict__ seeds64,
+                                     const void* __restrict__ init, int init_f64, int64_t n0, int64_t n,
+                                     uint32_t* __restrict__ flags) {
+  if ((flags[1] & 2u) == 0) return;
+  const int64_t first = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (first == 0) atomicAnd(&flags[0], ~1u);  // refined values are recomputed; seed failure stays
+  for (int64_t i = first; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    vals64[i] = i < n0 ? (seeds64 ? seeds64[i] : load_as<double>(init, init_f64, i)) : Sentinel<double>::value();
+}
+
+template <typename T>
+__global__ void fallback_copy_kernel(const double* __restrict__ vals64, T* __restrict__ vals, int64_t n0, int64_t n,
+                                     const uint32_t* __restrict__ flags) {
+  if ((flags[1] & 2u) == 0) return;
+  for (int64_t i = n0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    vals[i] = (T)vals64[i];
 }
 
 __global__ void nan_fill_kernel(void* __restrict__ out, int out_f64, int64_t n, const uint32_t* __restrict__ flags) {
@@ -416,6 +439,7 @@ struct RefineWs {
   // two-phase generate (FP32 increment mode)
   T* wts = nullptr;
   T* sig = nullptr;
+  double* vals64 = nullptr;  // the FP64 fallback's values
   int32_t* fixslot = nullptr;
   uint32_t* fix_list = nullptr;
   uint32_t fix_cap = 0;
@@ -466,6 +490,7 @@ bool carve(Workspace& w, const RefineProblem& p, int mode, RefineWs<T>& b) {
     b.fix_list = w.take<uint32_t>(b.fix_cap);
     b.w64 = w.take<double>((size_t)b.fix_cap * p.k);
     b.sig64 = w.take<double>(b.fix_cap);
+    b.vals64 = w.take<double>(p.n);
   }
   return w.ok();
 }
@@ -622,6 +647,51 @@ int run_two_phase(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hipSt
   return kOk;
 }
 
+// The FP64 fallback of the two-phase generate (see fallback_fill_kernel): three launches
+// that return at once unless the FP64 list overflowed.
+template <typename T>
+int run_fallback(const RefineArgs<T>& a, const RefineProblem& p, const RefineIO& io, RefineWs<T>& b, hipStream_t s) {
+  StageScope stage("fp64_fallback", s);
+  const int64_t n0 = p.n - p.n_ref;
+  GGP_TRY(fill_u32(b.ctr, 0u, 1, s));  // fresh ticket
+  hipLaunchKernelGGL(fallback_fill_kernel, dim3(grid_for(std::min<int64_t>(p.n, 1 << 20), 256)), dim3(256), 0, s,
+                     b.vals64, p.with_seed ? (const double*)b.dout : nullptr, io.init, io.init_f64, n0, p.n,
+                     b.ctr + 1);
+  GGP_LAUNCH_CHECK();
+  RefineArgs<double> f{};
+  f.pts = p.sched ? p.sched_pts : p.pts;
+  f.pts_f64 = p.pts_f64;
+  f.d = p.d;
+  f.nbrs = p.sched ? p.sched_nbrs : p.nbrs;
+  f.k = p.k;
+  f.n0 = n0;
+  f.n_ref = p.n_ref;
+  f.perm = p.perm;
+  f.xi = io.xi;
+  f.xi_f64 = io.xi_f64;
+  f.vals = b.vals64;
+  f.out = a.out;  // null for the raw refine, whose graph-order values are copied below
+  f.out_f64 = io.out_f64;
+  f.io_base = a.io_base;
+  f.tab = b.tab64;
+  f.meta = b.meta64;
+  f.ticket = b.ctr;
+  f.flags = b.ctr + 1;
+  f.mode = kGenerate;
+  f.increment = 0;
+  f.sched = p.sched;
+  f.only_if = b.ctr + 2;
+  const size_t lds64 = (size_t)(p.nb - 1) * sizeof(TabE<double>) + (size_t)p.nb * sizeof(double);
+  f.fix_lds_offset = (int)((lds64 + 15) & ~size_t(15));
+  GGP_TRY(launch_refine<double>(f, p.k + 1, s, lds64, device_cus() * 4));
+  if (!p.with_seed) {
+    hipLaunchKernelGGL(fallback_copy_kernel<T>, dim3(grid_for(std::min<int64_t>(p.n_ref, 1 << 20), 256)), dim3(256),
+                       0, s, b.vals64, static_cast<T*>(io.out), n0, p.n, b.ctr + 1);
+    GGP_LAUNCH_CHECK();
+  }
+  return kOk;
+}
+
 template <typename T>
 int refine_flow(const RefineProblem& p, int mode, const RefineIO& io, void* ws, size_t ws_bytes, hipStream_t s) {
   GGP_CHECK_ARG(p.n >= 1 && p.n_ref >= 0 && p.n_ref <= p.n && p.k >= 1 && p.d >= 1, "refine: bad sizes");
@@ -677,6 +747,7 @@ int refine_flow(const RefineProblem& p, int mode, const RefineIO& io, void* ws, 
     a.out_f64 = io.out_f64;
     if (two_phase<T>(p, mode)) {
       GGP_TRY(run_two_phase<T>(a, p, b, s));
+      GGP_TRY(run_fallback<T>(a, p, io, b, s));
     } else if (p.sched != nullptr && !generic_config(p)) {
       // the fused dataflow in slot space: points and neighbours in schedule order, values
       // stored in slot order, xi and output through the schedule

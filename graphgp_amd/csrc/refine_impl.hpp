@@ -148,6 +148,9 @@ struct RefineArgs {
   // n0 + q
   const int32_t* sched;
   const int32_t* nbv;
+  // a launch that runs only if bit 2 of *only_if is set (the FP64 fallback of the
+  // two-phase generate); null: always
+  const uint32_t* only_if;
 };
 
 // ------------------------------------------------------------------ helpers
@@ -464,6 +467,8 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
   __shared__ int s_fixlist[GPT];
   __shared__ double s_fixsig[GPT];
 
+  if (a.only_if != nullptr && (__hip_atomic_load(a.only_if, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 2u) == 0)
+    return;
   const TabMeta meta = *a.meta;
   // two variants are launched; the one that does not match the table exits at once
   if ((meta.log_exact != 0) != FAST) return;

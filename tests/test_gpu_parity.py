@@ -543,3 +543,29 @@ def test_scheduled_generate_equals_unscheduled(monkeypatch):
     monkeypatch.setattr(R, "SCHEDULE_MIN_REFINED", 1 << 40)
     b = gp.generate(g, cov, xi, precision="fp32_increment")
     assert float((a - b).abs().max()) <= 1e-4
+
+
+@pytest.mark.parametrize("n", [60_000, 1_100_000])
+def test_fp32_increment_list_overflow_falls_back_to_fp64(n, monkeypatch):
+    # When more points need the FP64 recomputation than its list holds (a quarter of them),
+    # generate recomputes the whole refinement in FP64 and returns those values, rounded.
+    # A threshold of 1 flags every point.  1.1M points run with the dataflow schedule (slot
+    # space), 60k points in graph order; the raw refine takes the same path.
+    p = clustered_points(n, d=3, seed=71)
+    g = gp.build_graph(p, n0=200, k=16)
+    cov = gp.extras.matern_kernel(p=1, variance=1.0, cutoff=0.05, r_min=1e-6, r_max=2.0, n_bins=2001, jitter=1e-4)
+    xi = torch.as_tensor(np.random.default_rng(72).standard_normal(n), dtype=torch.float32, device="cuda")
+    ref = gp.generate(g, cov, xi, precision="fp64")
+    out, cnt = gp.generate(g, cov, xi, precision="fp32_increment", return_counters=True)
+    assert cnt["fp64_list_overflow"] == 0 and cnt["fp64_recomputed"] < (n - 200) // 4
+    monkeypatch.setenv("GGP_FIX_TAU", "1")
+    out, cnt = gp.generate(g, cov, xi, precision="fp32_increment", return_counters=True)
+    assert cnt["fp64_list_overflow"] == 1 and cnt["nan"] == 0
+    assert torch.equal(out, ref.float())
+    # raw refine (graph order, seeds given): the refined rows are the FP64 refinement's
+    n0 = g.offsets[0]
+    gi = g.indices.long()
+    init = ref[gi[:n0]].float()
+    raw = gp.refine(g.points, g.neighbors, g.offsets, cov, init, xi[gi[n0:]], precision="fp32_increment")
+    raw64 = gp.refine(g.points, g.neighbors, g.offsets, cov, init.double(), xi[gi[n0:]].double(), precision="fp64")
+    assert torch.equal(raw[n0:], raw64[n0:].float())
