@@ -457,7 +457,7 @@ bool two_phase(const RefineProblem& p, int mode) {
 }
 
 // list slots for the FP64 recomputation: a quarter of the points (the workloads measured here
-// flag about 1%); beyond that the extra points keep their FP32 weights and error bit 2 is set
+// flag about 1.4%); beyond that error bit 2 is set and the FP64 fallback recomputes everything
 inline uint32_t fix_capacity(int64_t n_ref) { return (uint32_t)std::max<int64_t>(4096, n_ref / 4); }
 
 template <typename T>

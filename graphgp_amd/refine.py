@@ -10,8 +10,11 @@ compatibility; there is one code path.
   * ``"fp32"``           -- FP32, the reference formulation;
   * ``"fp32_increment"`` -- FP32 in increments relative to the nearest neighbour (keeps the
                             relative precision of the covariance entries when points cluster);
-                            points whose FP32 pivots or Schur complement fall below 1e-2 of
-                            their diagonal are recomputed in FP64 inside the same kernel.
+                            points whose FP32 pivots or Schur complement fall below 1e-3 of
+                            their diagonal are recomputed in FP64 (generate: by a separate
+                            pass over the list of such points, and if they are more than a
+                            quarter of all points, by the FP64 refinement of every point;
+                            inverse and log-det: inside the same kernel).
 The default is ``"fp64"`` when the floating-point input (xi / values / points) is float64,
 else ``"fp32"``.  Outputs have the compute dtype.
 """
@@ -47,8 +50,8 @@ def _counters(ws):
     buf = (ctypes.c_uint32 * 4)()
     _lib.check(_lib.lib().ggp_refine_counters(_lib.ptr(ws), buf, _lib.stream()), "refine counters")
     # error word bits: 1 = a wait gave up (malformed graph), 2 = more FP32-increment points
-    # needed the FP64 recomputation than its list holds (a quarter of the points); the rest
-    # kept their FP32 weights
+    # needed the FP64 recomputation than its list holds (a quarter of the points), and the
+    # generate returned the FP64 refinement's values instead
     return {"tiles": int(buf[0]), "nan": int(buf[1]), "malformed": int(buf[2] & 1),
             "fp64_list_overflow": int((buf[2] >> 1) & 1), "fp64_recomputed": int(buf[3])}
 
