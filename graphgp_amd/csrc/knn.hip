@@ -22,11 +22,12 @@
 //
 // MI355X mapping: the k slots live in VGPRs (k <= 64, compile-time slot count KMAX with
 // slots >= k pinned at -inf so they never hold the maximum); the final order is a
-// register bitonic network.  Larger k uses slots in a caller workspace.  Each visit issues
-// all of its loads at once (the node's point and split, both children's boxes), so it
-// costs one memory round trip; up to k = 32 the kernel is held to three wavefronts per SIMD
-// (168 VGPRs).  At 16.8M clustered points, k = 30: 263 -> 228 ms against two round trips
-// per visit.
+// register bitonic network.  Larger k uses slots in a caller workspace.  The kernel is held
+// to five wavefronts per SIMD up to k = 16 and three up to k = 32 (96 and 168 VGPRs; the
+// compiler chose 103 and 169 VGPRs, one wavefront fewer each), and a visit's point and split
+// loads carry no branch: at 16.8M clustered points, k = 30, 263 -> 225 ms.  Reading both
+// children's boxes with the node (one round trip per visit instead of two) was measured and
+// dropped: no gain at 16.8M (229 ms), and 0.45 -> 0.65 s at 67M uniform points, k = 16.
 #include "common.hpp"
 #include "graphgp_amd_internal.hpp"
 
@@ -130,28 +131,14 @@ __device__ __forceinline__ void knn_query(const T* __restrict__ pts, const int32
   while (cur != -1) {
     const int64_t parent = tree_parent(cur);
     const int64_t l = tree_left(cur), r = tree_right(cur);
-    // Every load of this visit at once, so a visit costs one memory round trip: the node's
-    // point and split dimension, and the boxes of both children (which one is near is only
-    // known from the point; a child that does not exist reads the node's own box, unused).
-    // (coordinates j >= d, which the instantiation carries beyond the data's dimension,
-    // read coordinate d - 1 and are unused: a load under a branch would wait at its join)
+    // The node's point and split dimension, read together; coordinates j >= d, which the
+    // instantiation carries beyond the data's dimension, read coordinate d - 1 and are
+    // unused (a load under a branch would be waited for at its join).  A child's box is read
+    // when it is tested.
     Query<T, D> p;
 #pragma unroll
     for (int j = 0; j < D; ++j) p.c[j] = pts[cur * d + (j < d ? j : d - 1)];
     const int dim = split[cur];
-    T bl[2 * D], br[2 * D];
-    {
-      const T* bpl = box + (l < n ? l : cur) * 2 * d;
-      const T* bpr = box + (r < n ? r : cur) * 2 * d;
-#pragma unroll
-      for (int j = 0; j < D; ++j) {
-        const int jc = j < d ? j : d - 1;
-        bl[j] = bpl[jc];
-        bl[D + j] = bpl[d + jc];
-        br[j] = bpr[jc];
-        br[D + j] = bpr[d + jc];
-      }
-    }
     if (prev == parent) {
       const T s = dist2<T, D>(q, p.c, d);
       if (s < r2) {
@@ -170,20 +157,8 @@ __device__ __forceinline__ void knn_query(const T* __restrict__ pts, const int32
     const int64_t nearc = left_near ? l : r;
     const int64_t farc = left_near ? r : l;
     const bool far_in = sdist * sdist <= r2;
-    // box distance of the near and far child, from the registers loaded above
     auto child_box_dist2 = [&](bool near) -> T {
-      const bool left = near == left_near;
-      T s2 = T(0);
-#pragma unroll
-      for (int j = 0; j < D; ++j) {
-        if (j < d) {
-          const T blo = left ? bl[j] : br[j], bhi = left ? bl[D + j] : br[D + j];
-          const T lo = blo - q.c[j], hi = q.c[j] - bhi;
-          const T g = lo > T(0) ? lo : (hi > T(0) ? hi : T(0));
-          s2 = j == 0 ? g * g : s2 + g * g;
-        }
-      }
-      return s2;
+      return box_dist2_rt<T>(q.c, box + ((near == left_near) ? l : r) * 2 * d, d);
     };
     int64_t next;
     if (prev == nearc || (prev == parent && (nearc >= max_index || child_box_dist2(true) > r2)))
@@ -227,7 +202,8 @@ __device__ __forceinline__ void knn_query(const T* __restrict__ pts, const int32
 // launch capped below the query count, so that it leaves room for other work on the
 // device): workgroups take blocks of 256 queries from the ticket until none are left.
 template <typename T, int D, int KMAX, bool TICKET>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KMAX <= 32 ? 3 : 1))) void knn_kernel(const T* __restrict__ pts, const int32_t* __restrict__ split,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KMAX <= 16 ? 5 : KMAX <= 32 ? 3 : 1)))
+void knn_kernel(const T* __restrict__ pts, const int32_t* __restrict__ split,
                                                   int d, const int32_t* __restrict__ qidx,
                                                   const int32_t* __restrict__ midx, int64_t q0, int64_t nq, int k,
                                                   int32_t* __restrict__ out, int64_t n, const T* __restrict__ box,
