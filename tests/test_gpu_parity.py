@@ -440,23 +440,36 @@ def test_indefinite_seed_block_reference_rules():
 
 def test_logdet_with_indefinite_seed_block_is_finite():
     # generate_logdet = seed block's slogdet / 2 + sum of log std: finite although the seed
-    # block is indefinite; generate and generate_inv are all NaN
+    # block is indefinite; generate and generate_inv are all NaN.  The graph is assembled
+    # here so that its first 1000 points are exactly the indefinite seeds: 4000 refined
+    # points near them, each with its 10 nearest preceding points (brute force), in the
+    # reference's depth order (the oracle's depths, order and offsets).
     import os
 
-    seeds = np.load(os.path.join(os.path.dirname(__file__), "golden", "indefinite_seeds_4m.npy"))
+    seeds = np.load(os.path.join(os.path.dirname(__file__), "golden", "indefinite_seeds_4m.npy")).astype(np.float32)
     rng = np.random.default_rng(5)
     extra = (seeds[rng.integers(0, len(seeds), 4000)] + rng.normal(0, 0.01, (4000, 3))).astype(np.float32)
+    pts = np.concatenate([seeds, extra])
+    n0, k, n = len(seeds), 10, len(seeds) + len(extra)
+    nb = np.empty((n - n0, k), np.int32)
+    for r in range(n - n0):
+        d2 = ((pts[: n0 + r] - pts[n0 + r]) ** 2).sum(axis=1)
+        nb[r] = np.argsort(d2, kind="stable")[:k]
+    dep = O.compute_depths(nb, n0=n0)
+    po, io, no, do = O.order_by_depth(pts, np.arange(n), nb, dep)
+    assert np.array_equal(po[:n0], seeds)  # the seeds keep their places (depth 0)
+    g = gp.Graph(torch.from_numpy(po).cuda(), torch.from_numpy(no).cuda(), O.offsets_from_depths(do),
+                 torch.from_numpy(io.astype(np.int32)).cuda())
+    gp.check_graph(g)
     kw = dict(p=1, variance=1.0, cutoff=0.05, r_min=1e-6, r_max=2.0, n_bins=2001, jitter=1e-5)
     cov, ocov = gp.extras.matern_kernel(**kw), O.matern_kernel(**kw)
-    g = gp.build_graph(np.concatenate([seeds.astype(np.float32), extra]), n0=1000, k=10)
-    og = O.OGraph(cpu(g.points), cpu(g.neighbors), g.offsets, cpu(g.indices))
-    n0 = g.offsets[0]
-    sd = O.generate_dense_logdet(og.points[:n0], ocov)
-    if not np.isfinite(sd) or not np.all(np.isnan(O.generate_dense(og.points[:n0], ocov, np.ones(n0)))):
-        pytest.skip("the tree's first 1000 points form a positive-definite seed block")
+    og = O.OGraph(po, no, g.offsets, io)
+    sd = O.generate_dense_logdet(seeds, ocov)
+    assert np.isfinite(sd) and np.all(np.isnan(O.generate_dense(seeds, ocov, np.ones(n0))))
     ld = float(gp.generate_logdet(g, cov, precision="fp64"))
-    assert np.isclose(ld, O.generate_logdet(og, ocov), rtol=1e-9)
-    assert np.all(np.isnan(cpu(gp.generate(g, cov, np.ones(5000), precision="fp64"))))
+    assert np.isfinite(ld) and np.isclose(ld, O.generate_logdet(og, ocov), rtol=1e-9)
+    assert np.all(np.isnan(cpu(gp.generate(g, cov, np.ones(n), precision="fp64"))))
+    assert np.all(np.isnan(cpu(gp.generate_inv(g, cov, np.ones(n), precision="fp64"))))
 
 
 def test_argument_errors():
