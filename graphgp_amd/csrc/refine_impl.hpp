@@ -156,10 +156,16 @@ struct RefineArgs {
 // ------------------------------------------------------------------ helpers
 // FP32, four lanes per point: the trailing update of column COL in column MM (slot MS,
 // lane ML) for the lane's rows in slots MS..R-1, m[r][MM] -= m[r][COL] * (lane ML's
-// m[MS][COL]), as one statement of DPP-fused FMAs.
-template <int ML, int MS, int P, int COL, int MM, int S, int... Is>
+// m[MS][COL]), as one statement of DPP-fused FMAs.  Only the column's first statement
+// (MM = COL + 1) starts with s_nop: every source of the column, m[MS][COL], is also a
+// multiplicand of that first statement (its rows are all rows from slot (COL + 1) / P on),
+// so the column scaling that wrote it precedes the first statement, and the first
+// statement's s_nop and FMAs separate it from every later one.  The compiler sees the
+// statements' writes, so its own DPP reads after them (the next pivot) get their wait
+// states from the compiler.
+template <int ML, int MS, int P, int COL, int MM, bool NOP, int S, int... Is>
 __device__ __forceinline__ void update_rows_dpp(float (&m)[S], std::integer_sequence<int, Is...>) {
-  FmacDpp<ML, sizeof...(Is)>::run(m[P * MS * (MS + 1) / 2 + COL], m[P * (MS + Is) * (MS + Is + 1) / 2 + MM]...,
+  FmacDpp<ML, sizeof...(Is), NOP>::run(m[P * MS * (MS + 1) / 2 + COL], m[P * (MS + Is) * (MS + Is + 1) / 2 + MM]...,
                                   m[P * (MS + Is) * (MS + Is + 1) / 2 + COL]...);
 }
 
@@ -499,6 +505,17 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
   const int lane = threadIdx.x & (P - 1);
   const int grp = threadIdx.x / P;
   const int k_arg = a.k;
+  // The weights phase builds and factorises all KK rows and columns without run-time
+  // column tests: rows i > k (padding, zero coordinates) and columns j > k hold garbage,
+  // and in a right-looking Cholesky row i only ever updates rows and columns >= i, so
+  // they never reach the k + 1 real rows; only the pivot scale of columns >= k (which
+  // would rescale the fine point's Schur complement) and the recurrence's writes for
+  // i >= k are masked.  Without the tests (one scalar compare and branch per column and
+  // per update, a third of the instructions issued) the weights phase takes 11.5 ms
+  // instead of 14.4 ms at 16.8M points, k = 30.  The fused modes, the exact-lookup
+  // variant and FP64 keep the tests: without them the compiler's schedule of their single
+  // large block spills hundreds of VGPRs.
+  constexpr bool GUARD = !(MODE == kWeights && FAST && sizeof(T) == 4);
   // the weights phases run in one formulation each (FP32: increments; FP64 list: plain),
   // fixed at compile time there
   const bool incr = MODE == kWeights ? sizeof(T) == 4 : a.increment != 0;
@@ -582,7 +599,7 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
     static_for<0, KK>([&](auto jc) {
       constexpr int j = decltype(jc)::value;
       constexpr int js = j % P, jr = j / P;
-      if (j <= k) {
+      if (!GUARD || j <= k) {
         T cj[D];
 #pragma unroll
         for (int t = 0; t < D; ++t) cj[t] = bcast<P, js>(c[jr][t]);
@@ -629,18 +646,18 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
     static_for<0, KK - 1>([&](auto cc) {
       constexpr int col = decltype(cc)::value;
       constexpr int cs = col / P, cl = col % P;
-      if (col < k) {
+      if (!GUARD || col < k) {
         const T dg = bcast<P, cl>(m[P * cs * (cs + 1) / 2 + col]);
-        const T rs = rsqrt_t<T>(dg);
+        const T rs = (GUARD || col < k) ? rsqrt_t<T>(dg) : T(1);
         if (lane == cl) rdl[cs] = rs;
 #pragma unroll
         for (int r = cs; r < R; ++r) m[P * r * (r + 1) / 2 + col] *= rs;
         static_for<col + 1, KK>([&](auto mc) {
           constexpr int mm = decltype(mc)::value;
           constexpr int ms = mm / P, ml = mm % P;
-          if (mm <= k) {
+          if (!GUARD || mm <= k) {
             if constexpr (sizeof(T) == 4 && P == 4) {
-              update_rows_dpp<ml, ms, P, col, mm>(m, std::make_integer_sequence<int, R - ms>());
+              update_rows_dpp<ml, ms, P, col, mm, mm == col + 1>(m, std::make_integer_sequence<int, R - ms>());
             } else {
               const T b = bcast<P, ml>(m[P * ms * (ms + 1) / 2 + col]);
 #pragma unroll
@@ -706,7 +723,7 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
       static_for<0, KK - 1>([&](auto ic) {
         constexpr int i = KK - 2 - decltype(ic)::value;
         constexpr int is = i / P, il = i % P;
-        if (i < k) {
+        if (!GUARD || i < k) {
           T part = T(0);
 #pragma unroll
           for (int r = is; r < R; ++r) {
@@ -715,7 +732,7 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
           }
           part = group_sum<P>(part);
           const T wi = -part * bcast<P, il>(rdl[is]);
-          if (lane == il) wv[is] = wi;
+          if (lane == il && i < k) wv[is] = wi;
         }
       });
       if (valid && a.list_in) {
