@@ -567,11 +567,48 @@ int run_kernel_inner(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hi
   return rc;
 }
 
-// The three phases of generate in the FP32 increment mode (a.vals already holds the seeds
-// and sentinels): the weights kernel, the FP64 recomputation of the listed points, and the
-// dataflow over the stored weights.
+// A second stream per host thread and device, for the seed block that runs beside the
+// weights phase, and the events that fork it from and join it to the caller's stream.
+// Default priority, non-blocking; created on first use and kept for the thread's lifetime.
+struct SideStream {
+  hipStream_t s = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
+
+int side_stream(SideStream** out) {
+  thread_local SideStream side[16];
+  int dev = 0;
+  GGP_HIP(hipGetDevice(&dev));
+  SideStream& x = side[dev & 15];
+  if (x.s == nullptr) {
+    GGP_HIP(hipStreamCreateWithFlags(&x.s, hipStreamNonBlocking));
+    GGP_HIP(hipEventCreateWithFlags(&x.fork, hipEventDisableTiming));
+    GGP_HIP(hipEventCreateWithFlags(&x.join, hipEventDisableTiming));
+  }
+  *out = &x;
+  return kOk;
+}
+
+// Refined points from which the seed block of the two-phase generate runs beside the
+// weights phase, and the workgroup slots the weights phase leaves free for it meanwhile:
+// one per eight CUs (32 on MI355X; at 16.8M points, k = 30, the seed block then takes 1.5 ms
+// beside the weights phase, with 16 free slots 13.4 ms).  GGP_SEED_RESERVE overrides it;
+// 0 runs the seed block before the weights phase.
+constexpr int64_t kSeedBesideMinRefined = int64_t(1) << 20;
+int seed_reserve_wgs() {
+  const char* e = getenv("GGP_SEED_RESERVE");
+  return e ? atoi(e) : std::max(1, device_cus() / 8);
+}
+
+// The three phases of generate in the FP32 increment mode: the weights kernel, the FP64
+// recomputation of the listed points, and the dataflow over the stored weights.  a.vals
+// already holds the seeds and sentinels, or the seed block is running on `side`: then the
+// weights kernel leaves seed_reserve_wgs() workgroup slots free for it, a second launch of
+// the weights kernel with that many workgroups follows the seed block on the side stream
+// and takes the tiles left by then from the same ticket, and the FP64 list waits for both.
 template <typename T>
-int run_two_phase(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hipStream_t s) {
+int run_two_phase(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hipStream_t s,
+                  SideStream* side = nullptr) {
   if (g_prof_start) GGP_HIP(hipEventRecord(g_prof_start, s));
   setup_args<T>(a, p, b);  // the later phases launch with this copy of the arguments
   std::optional<StageScope> stage;
@@ -586,7 +623,18 @@ int run_two_phase(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hipSt
   // the weights phases store sigma xi per slot (xi through the schedule and the permutation)
   a.sched = p.sched;
   a.mode = kWeights;
-  GGP_TRY(run_kernel_inner<T>(a, pw, b, s));
+  if (side != nullptr) {
+    a.reserve_wgs = seed_reserve_wgs();
+    GGP_TRY(run_kernel_inner<T>(a, pw, b, s));
+    a.reserve_wgs = 0;
+    a.grid_limit = seed_reserve_wgs();
+    GGP_TRY(run_kernel_inner<T>(a, pw, b, side->s));
+    a.grid_limit = 0;
+    GGP_HIP(hipEventRecord(side->join, side->s));
+    GGP_HIP(hipStreamWaitEvent(s, side->join, 0));  // all weights tiles, and the seeds
+  } else {
+    GGP_TRY(run_kernel_inner<T>(a, pw, b, s));
+  }
   // the profiling events bracket the dominant kernel: the weights phase
   if (g_prof_stop) GGP_HIP(hipEventRecord(g_prof_stop, s));
   stage.emplace("fp64_list", s);
@@ -719,17 +767,29 @@ int refine_flow(const RefineProblem& p, int mode, const RefineIO& io, void* ws, 
   a.perm = p.perm;
   if (mode == kGenerate) {
     // graph-order buffer: seeds (given or from the dense block) then sentinels
+    // the seed block of a large two-phase generate runs on the side stream beside the
+    // weights phase (which needs no seed); the dataflow waits for it
+    SideStream* side = nullptr;
+    if (p.with_seed && two_phase<T>(p, mode) && p.n_ref >= kSeedBesideMinRefined && cu_budget() == 0 &&
+        yield_cfg().dev == nullptr && seed_reserve_wgs() > 0)
+      GGP_TRY(side_stream(&side));
     if (p.with_seed) {
-      StageScope stage("seed_block", s);
       hipLaunchKernelGGL(seed_fill_kernel<T>, dim3(grid_for(p.n, 256)), dim3(256), 0, s, b.vals, (const void*)nullptr,
                          0, (int64_t)0, p.n);
       GGP_LAUNCH_CHECK();
-      GGP_TRY(dense_factor(p.pts, p.pts_f64, p.d, n0, p.bins, p.vals, p.nb, b.dense, s));
-      hipLaunchKernelGGL(gather_seed_xi_kernel, dim3(grid_for(n0, 256)), dim3(256), 0, s, io.xi, io.xi_f64, p.perm,
+      hipStream_t ss = s;
+      if (side != nullptr) {
+        GGP_HIP(hipEventRecord(side->fork, s));
+        GGP_HIP(hipStreamWaitEvent(side->s, side->fork, 0));
+        ss = side->s;
+      }
+      StageScope stage("seed_block", ss);
+      GGP_TRY(dense_factor(p.pts, p.pts_f64, p.d, n0, p.bins, p.vals, p.nb, b.dense, ss));
+      hipLaunchKernelGGL(gather_seed_xi_kernel, dim3(grid_for(n0, 256)), dim3(256), 0, ss, io.xi, io.xi_f64, p.perm,
                          n0, b.dvec);
       GGP_LAUNCH_CHECK();
-      GGP_TRY(dense_apply(b.dense, n0, b.dvec, b.dout, s));
-      hipLaunchKernelGGL(seed_out_kernel<T>, dim3(grid_for(n0, 256)), dim3(256), 0, s, b.dout, p.perm, n0, b.vals,
+      GGP_TRY(dense_apply(b.dense, n0, b.dvec, b.dout, ss));
+      hipLaunchKernelGGL(seed_out_kernel<T>, dim3(grid_for(n0, 256)), dim3(256), 0, ss, b.dout, p.perm, n0, b.vals,
                          io.out, io.out_f64, b.ctr + 1);
       GGP_LAUNCH_CHECK();
       a.vals = b.vals;
@@ -748,7 +808,7 @@ int refine_flow(const RefineProblem& p, int mode, const RefineIO& io, void* ws, 
     a.xi_f64 = io.xi_f64;
     a.out_f64 = io.out_f64;
     if (two_phase<T>(p, mode)) {
-      GGP_TRY(run_two_phase<T>(a, p, b, s));
+      GGP_TRY(run_two_phase<T>(a, p, b, s, side));
       GGP_TRY(run_fallback<T>(a, p, io, b, s));
     } else if (p.sched != nullptr && !generic_config(p)) {
       // the fused dataflow in slot space: points and neighbours in schedule order, values

@@ -151,6 +151,10 @@ struct RefineArgs {
   // a launch that runs only if bit 2 of *only_if is set (the FP64 fallback of the
   // two-phase generate); null: always
   const uint32_t* only_if;
+  // workgroup slots a persistent launch leaves free (for the seed block running beside it),
+  // and a cap on its grid (0: none)
+  int reserve_wgs;
+  int grid_limit;
 };
 
 // ------------------------------------------------------------------ helpers

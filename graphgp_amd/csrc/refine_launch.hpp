@@ -17,6 +17,11 @@ int launch_variant(const RefineArgs<T>& a, hipStream_t s, size_t lds_bytes, int 
     }
   }
   grid = persistent_grid((const void*)kern, grid, lds_bytes);
+  if (a.reserve_wgs > 0) {
+    const int cap = blocks_per_cu((const void*)kern, lds_bytes) * device_cus() - a.reserve_wgs;
+    if (cap >= 1 && grid > cap) grid = cap;
+  }
+  if (a.grid_limit > 0 && grid > a.grid_limit) grid = a.grid_limit;
   RefineArgs<T> b = a;
   b.yield = yield_cfg().dev;
   b.keep = yield_keep(grid);
