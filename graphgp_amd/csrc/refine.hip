@@ -74,9 +74,9 @@ __global__ void prep_packed_kernel(const double* __restrict__ bins, const double
   if (j >= nb) return;
   const double f0 = vals[0];
   TabP e;
-  e.pad = 0.f;
   if (j == nb - 1) {
     e.c = (float)(increment ? f0 - vals[nb - 1] : vals[nb - 1]);
+    e.i = e.c;
     e.s = 0.f;
     e.e = (float)bins[nb - 1];
   } else {
@@ -90,6 +90,7 @@ __global__ void prep_packed_kernel(const double* __restrict__ bins, const double
     e.c = (float)fa;
     e.s = (float)sl;
     e.e = (float)bins[j];
+    e.i = (float)(fa - sl * (double)e.e);
   }
   tab[j] = e;
 }

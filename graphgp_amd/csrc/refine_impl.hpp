@@ -38,11 +38,12 @@ __device__ __forceinline__ int log_guess(float r2, float lb1, float inv_step) {
 }
 
 // The FP32 table, packed: entry j < nseg is segment j on [e_j, e_{j+1}) with value
-// c + (r - e) s (c the value at e, so r - e is exact and the entry keeps its relative
-// precision); entry nseg is the constant tail {f_last, 0, x_last} for r >= x_last.  One
-// 16-byte LDS read returns a segment's line and edge.
+// c + (r - e) s (c the value at e, so r - e is exact) = i + r s (i = c - e s, the
+// intercept, rounded from FP64); entry nseg is the constant tail {f_last, 0, x_last,
+// f_last} for r >= x_last.  The log-spaced lookup reads {i, s} (one 8-byte LDS read, one
+// FMA); the exact lookup reads the whole entry.
 struct alignas(16) TabP {
-  float c, s, e, pad;
+  float i, s, e, c;
 };
 
 // Segment of r (given r^2) in a log-spaced packed table: t = 1 + (log2(r) - log2(x_1)) /
@@ -58,8 +59,8 @@ __device__ __forceinline__ int packed_index(float r2, float ka, float kb, int ns
 // (any monotone bins); both land on the tail entry for r >= x_last.
 __device__ __forceinline__ float packed_lookup_fast(const TabP* __restrict__ tab, float ka, float kb, int nseg,
                                                     float r2) {
-  const TabP e = tab[packed_index(r2, ka, kb, nseg)];
-  return e.c + (__builtin_amdgcn_sqrtf(r2) - e.e) * e.s;
+  const TabP& e = tab[packed_index(r2, ka, kb, nseg)];
+  return __builtin_fmaf(__builtin_amdgcn_sqrtf(r2), e.s, e.i);
 }
 __device__ __forceinline__ float packed_lookup(const TabP* __restrict__ tab, int nseg, float r) {
   int lo = 0, hi = nseg;
@@ -620,11 +621,11 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
             r2v[r] = pair_dist2<T, D>(c[r], cj);
             ix[r] = packed_index(r2v[r], ka, kb, nseg);
           }
-          TabP e[R];
+          float2 e[R];
 #pragma unroll
-          for (int r = jr; r < R; ++r) e[r] = ptab[ix[r]];
+          for (int r = jr; r < R; ++r) e[r] = *reinterpret_cast<const float2*>(&ptab[ix[r]]);
 #pragma unroll
-          for (int r = jr; r < R; ++r) tv[r] = e[r].c + (__builtin_amdgcn_sqrtf(r2v[r]) - e[r].e) * e[r].s;
+          for (int r = jr; r < R; ++r) tv[r] = __builtin_fmaf(__builtin_amdgcn_sqrtf(r2v[r]), e[r].y, e[r].x);
         } else {
 #pragma unroll
           for (int r = jr; r < R; ++r) tv[r] = lookup2(pair_dist2<T, D>(c[r], cj));
