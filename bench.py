@@ -581,6 +581,8 @@ def main():
         "build_graph_ms": round(build_s * 1e3, 1),
         "build_stages": build_stages,
         "stages": stage_lines,
+        "stages_note": ("fp32_increment: the seed block runs on a second stream beside the weights phase "
+                        "(its time overlaps the weights stage); weights spans both weights launches"),
         "roofline": roofline,
         "cpu_baseline": cpu_baseline,
         "serve": serve,

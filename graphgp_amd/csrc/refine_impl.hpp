@@ -40,8 +40,10 @@ __device__ __forceinline__ int log_guess(float r2, float lb1, float inv_step) {
 // The FP32 table, packed: entry j < nseg is segment j on [e_j, e_{j+1}) with value
 // c + (r - e) s (c the value at e, so r - e is exact) = i + r s (i = c - e s, the
 // intercept, rounded from FP64); entry nseg is the constant tail {f_last, 0, x_last,
-// f_last} for r >= x_last.  The log-spaced lookup reads {i, s} (one 8-byte LDS read, one
-// FMA); the exact lookup reads the whole entry.
+// f_last} for r >= x_last.  The log-spaced lookup keeps only {i, s} in LDS, 8 bytes a
+// segment (one 8-byte LDS read and one FMA per lookup, and twice as many distinct LDS
+// banks among a wavefront's random segments as with 16-byte entries); the exact lookup
+// keeps the whole entry.
 struct alignas(16) TabP {
   float i, s, e, c;
 };
@@ -57,10 +59,10 @@ __device__ __forceinline__ int packed_index(float r2, float ka, float kb, int ns
 
 // Packed FP32 lookups: the log-spaced guess, and an exact binary search over the edges
 // (any monotone bins); both land on the tail entry for r >= x_last.
-__device__ __forceinline__ float packed_lookup_fast(const TabP* __restrict__ tab, float ka, float kb, int nseg,
+__device__ __forceinline__ float packed_lookup_fast(const float2* __restrict__ tab, float ka, float kb, int nseg,
                                                     float r2) {
-  const TabP& e = tab[packed_index(r2, ka, kb, nseg)];
-  return __builtin_fmaf(__builtin_amdgcn_sqrtf(r2), e.s, e.i);
+  const float2 e = tab[packed_index(r2, ka, kb, nseg)];
+  return __builtin_fmaf(__builtin_amdgcn_sqrtf(r2), e.y, e.x);
 }
 __device__ __forceinline__ float packed_lookup(const TabP* __restrict__ tab, int nseg, float r) {
   int lo = 0, hi = nseg;
@@ -472,6 +474,7 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
   constexpr bool PACKED = sizeof(T) == 4;  // FP32 reads the packed table
   TabE<T>* tab = reinterpret_cast<TabE<T>*>(smem_raw);
   TabP* ptab = reinterpret_cast<TabP*>(smem_raw);
+  float2* qtab = reinterpret_cast<float2*>(smem_raw);  // FP32 log-spaced: {intercept, slope}
   __shared__ uint32_t s_tile;
   __shared__ double s_red[4];
   __shared__ int s_nfix;
@@ -485,7 +488,11 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
   if ((meta.log_exact != 0) != FAST) return;
   T* edges = reinterpret_cast<T*>(tab + meta.nseg);
   if constexpr (PACKED) {
-    for (int j = threadIdx.x; j <= meta.nseg; j += blockDim.x) ptab[j] = a.tabp[j];
+    if constexpr (FAST) {
+      for (int j = threadIdx.x; j <= meta.nseg; j += blockDim.x) qtab[j] = make_float2(a.tabp[j].i, a.tabp[j].s);
+    } else {
+      for (int j = threadIdx.x; j <= meta.nseg; j += blockDim.x) ptab[j] = a.tabp[j];
+    }
   } else {
     const T* gedges = reinterpret_cast<const T*>(a.tab + meta.nseg);
     for (int j = threadIdx.x; j < meta.nseg; j += blockDim.x) tab[j] = a.tab[j];
@@ -497,7 +504,7 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
   const float lb1 = meta.lb1, inv_step = meta.inv_step, ka = meta.ka, kb = meta.kb;
   auto lookup2 = [&](T r2) -> T {
     if constexpr (PACKED) {
-      if constexpr (FAST) return packed_lookup_fast(ptab, ka, kb, nseg, r2);
+      if constexpr (FAST) return packed_lookup_fast(qtab, ka, kb, nseg, r2);
       else return packed_lookup(ptab, nseg, sqrt(r2));
     } else if constexpr (FAST) {
       const T r = fast_sqrt<T>(r2);
@@ -623,7 +630,7 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
           }
           float2 e[R];
 #pragma unroll
-          for (int r = jr; r < R; ++r) e[r] = *reinterpret_cast<const float2*>(&ptab[ix[r]]);
+          for (int r = jr; r < R; ++r) e[r] = qtab[ix[r]];
 #pragma unroll
           for (int r = jr; r < R; ++r) tv[r] = __builtin_fmaf(__builtin_amdgcn_sqrtf(r2v[r]), e[r].y, e[r].x);
         } else {
