@@ -69,19 +69,19 @@ def algorithmic_flops(k, d):
 # Algorithmic bytes per unit of each timed stage (the unit is a refined point unless noted):
 # the data the stage must read or write once, with neighbour coordinates and values counted
 # per gather.  The weights phase reads the k + 1 points' coordinates and the k indices and
-# writes k weights, sigma and the recomputation slot; the FP64 list (per recomputed point)
-# reads the same and writes k FP64 weights and sigma; the dataflow reads indices, weights,
-# slot, schedule, permutation, xi, sigma and the k neighbour values and writes the value and
-# the output.  Build stages (per point): the tree reads the points and writes them with the
+# writes k weights, sigma xi, the recomputation slot and the output position; the FP64
+# list (per recomputed point) reads the same and writes k FP64 weights and sigma xi; the
+# dataflow reads indices, weights, slot, output position, sigma xi and the k neighbour
+# values and writes the value and the output.  Build stages (per point): the tree reads the points and writes them with the
 # permutation and split dimensions; the kNN reads its query and writes k indices; depths
 # read the k indices and write a depth; the ordering reads and writes points, index,
 # neighbours and depth; the schedule (per refined point) reads indices and coordinates and
 # writes them in slot order with the slot.
 def stage_bytes(stage, k, d):
     return {
-        "weights": (k + 1) * d * 4 + 4 * k + 4 * k + 8,
+        "weights": (k + 1) * d * 4 + 4 * k + 4 * k + 12,
         "fp64_list": (k + 1) * d * 4 + 4 * k + 4 + 8 * k + 8,
-        "dataflow": 12 * k + 28,
+        "dataflow": 12 * k + 20,
         "refine": algorithmic_bytes(k, d, 4, 8),
         "tree": 2 * d * 4 + 8,
         "knn": d * 4 + 4 * k,

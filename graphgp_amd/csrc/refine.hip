@@ -442,6 +442,7 @@ struct RefineWs {
   T* sig = nullptr;
   double* vals64 = nullptr;  // the FP64 fallback's values
   int32_t* fixslot = nullptr;
+  int32_t* dpos = nullptr;
   uint32_t* fix_list = nullptr;
   uint32_t fix_cap = 0;
   double* w64 = nullptr;
@@ -488,6 +489,7 @@ bool carve(Workspace& w, const RefineProblem& p, int mode, RefineWs<T>& b) {
     b.wts = w.take<T>((size_t)p.n_ref * p.k);
     b.sig = w.take<T>(p.n_ref);
     b.fixslot = w.take<int32_t>(p.n_ref);
+    b.dpos = w.take<int32_t>(p.n_ref);
     b.fix_list = w.take<uint32_t>(b.fix_cap);
     b.w64 = w.take<double>((size_t)b.fix_cap * p.k);
     b.sig64 = w.take<double>(b.fix_cap);
@@ -539,6 +541,7 @@ void setup_args(RefineArgs<T>& a, const RefineProblem& p, RefineWs<T>& b) {
   a.wts = b.wts;
   a.sig = b.sig;
   a.fixslot = b.fixslot;
+  a.dpos = b.dpos;
   a.fix_list = b.fix_list;
   a.fix_cap = b.fix_cap;
   a.w64 = b.w64;

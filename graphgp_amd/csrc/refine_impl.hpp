@@ -126,6 +126,11 @@ struct RefineArgs {
   T* wts;
   T* sig;
   int32_t* fixslot;
+  // output position of every slot (the caller-order index through the schedule and the
+  // permutation), stored by the weights phase, which gathers it for xi anyway, so the
+  // dataflow reads it in slot order instead of gathering the schedule and the permutation;
+  // null: the dataflow gathers them
+  int32_t* dpos;
   uint32_t* fix_list;
   uint32_t fix_cap;
   double* w64;
@@ -578,7 +583,9 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
     T xw = T(0);
     if (mode == kWeights && valid) {
       const int64_t g = a.n0 + (a.sched != nullptr ? (int64_t)a.sched[pt] : pt);
-      xw = load_as<T>(a.xi, a.xi_f64, (a.perm != nullptr ? (int64_t)a.perm[g] : g) - a.io_base);
+      const int64_t gi = a.perm != nullptr ? (int64_t)a.perm[g] : g;
+      xw = load_as<T>(a.xi, a.xi_f64, gi - a.io_base);
+      if (a.dpos != nullptr && lane == 0) a.dpos[pt] = (int32_t)gi;
     }
 
     // ---- gather this lane's rows: neighbours first, fine point at row k
@@ -969,7 +976,8 @@ __global__ __launch_bounds__(256) void apply_kernel(RefineArgs<T> a) {
     if (tile >= a.ntiles) break;
     const int64_t q = tile * GPT + grp;  // slot: the refined point's index, or its schedule slot
     const bool valid = q < a.n_ref;
-    const int64_t pt = (a.sched != nullptr && valid) ? (int64_t)a.sched[q] : q;
+    // the refined point (graph order); with dpos only its output position is needed
+    const int64_t pt = (a.sched != nullptr && a.dpos == nullptr && valid) ? (int64_t)a.sched[q] : q;
     const int64_t fine = a.n0 + pt;  // graph position: output through perm
     const int64_t store = a.n0 + q;  // value slot; weights, sigma xi and fixslot are slot-indexed
     const int32_t slot = valid ? a.fixslot[q] : -1;
@@ -991,7 +999,7 @@ __global__ __launch_bounds__(256) void apply_kernel(RefineArgs<T> a) {
     int64_t dst = fine;
     const bool owner = lane == 0;
     if (valid && owner) {
-      dst = a.perm ? (int64_t)a.perm[fine] : fine;
+      dst = a.dpos != nullptr ? (int64_t)a.dpos[q] : a.perm ? (int64_t)a.perm[fine] : fine;
       sx = slot >= 0 ? (T)0 : a.sig[q];
     }
     T u[RM];
