@@ -464,7 +464,12 @@ inline uint32_t fix_capacity(int64_t n_ref) { return (uint32_t)std::max<int64_t>
 
 template <typename T>
 bool carve(Workspace& w, const RefineProblem& p, int mode, RefineWs<T>& b) {
-  b.ctr = w.take<uint32_t>(4);  // first, so ggp_refine_counters finds it at offset 0
+  // counters, first, so ggp_refine_counters finds them at offset 0: [0] ticket, [1] and [2]
+  // flags, [3] FP64-recomputed count, then the tickets of the two-phase generate's FP64 list
+  // [4] and FP64 fallback [5]; the dataflow's group tickets follow, so that one fill at the
+  // start clears them all
+  b.ctr = w.take<uint32_t>(8);
+  if (two_phase<T>(p, mode)) b.gtk = w.take<uint32_t>(kMaxGroups * 32);
   b.tab = w.take<TabE<T>>(p.nb);  // nb-1 segments + nb edges (of T) fit in nb entries + nb/2
   w.take<T>(p.nb + 2);
   if (sizeof(T) == 4) b.tabp = w.take<TabP>(p.nb);
@@ -484,7 +489,6 @@ bool carve(Workspace& w, const RefineProblem& p, int mode, RefineWs<T>& b) {
   }
   if (generic_config(p)) b.scratch = w.take<T>(generic_scratch_bytes<T>(p.k) / sizeof(T));
   if (two_phase<T>(p, mode)) {
-    b.gtk = w.take<uint32_t>(kMaxGroups * 32);
     b.fix_cap = (uint32_t)std::min<int64_t>(fix_capacity(p.n_ref), p.n_ref);
     b.wts = w.take<T>((size_t)p.n_ref * p.k);
     b.sig = w.take<T>(p.n_ref);
@@ -645,7 +649,6 @@ int run_two_phase(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hipSt
   // FP64 recomputation of the listed points with the FP64 register kernel (weights mode over
   // the list, FP64 direct table)
   {
-    GGP_TRY(fill_u32(b.ctr, 0u, 1, s));  // fresh ticket
     RefineArgs<double> d{};
     d.pts = w_pts;
     d.pts_f64 = p.pts_f64;
@@ -656,7 +659,7 @@ int run_two_phase(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hipSt
     d.n_ref = p.n_ref;
     d.tab = b.tab64;
     d.meta = b.meta64;
-    d.ticket = b.ctr;
+    d.ticket = b.ctr + 4;  // cleared at the start
     d.flags = b.ctr + 1;
     d.mode = kWeights;
     d.increment = 0;
@@ -676,7 +679,6 @@ int run_two_phase(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hipSt
   }
   const int cus = device_cus();
   stage.emplace("dataflow", s);
-  GGP_TRY(fill_u32(b.ctr, 0u, 1, s));  // fresh ticket for the dataflow
   a.mode = kGenerate;
   a.pts = p.pts;
   a.nbrs = p.nbrs;
@@ -694,8 +696,7 @@ int run_two_phase(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hipSt
   a.groups = std::min(grid, kMaxGroups);
   // every group keeps a workgroup that does not yield (the first `keep` ones do not)
   a.keep = std::max<uint32_t>(yield_keep(grid), (uint32_t)a.groups);
-  a.group_ticket = b.gtk;
-  GGP_TRY(fill_u32(b.gtk, 0u, kMaxGroups * 32, s));
+  a.group_ticket = b.gtk;  // cleared at the start
   hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, s, a);
   GGP_LAUNCH_CHECK();
   return kOk;
@@ -707,7 +708,6 @@ template <typename T>
 int run_fallback(const RefineArgs<T>& a, const RefineProblem& p, const RefineIO& io, RefineWs<T>& b, hipStream_t s) {
   StageScope stage("fp64_fallback", s);
   const int64_t n0 = p.n - p.n_ref;
-  GGP_TRY(fill_u32(b.ctr, 0u, 1, s));  // fresh ticket
   hipLaunchKernelGGL(fallback_fill_kernel, dim3(grid_for(std::min<int64_t>(p.n, 1 << 20), 256)), dim3(256), 0, s,
                      b.vals64, p.with_seed ? (const double*)b.dout : nullptr, io.init, io.init_f64, n0, p.n,
                      b.ctr + 1);
@@ -729,7 +729,7 @@ int run_fallback(const RefineArgs<T>& a, const RefineProblem& p, const RefineIO&
   f.io_base = a.io_base;
   f.tab = b.tab64;
   f.meta = b.meta64;
-  f.ticket = b.ctr;
+  f.ticket = b.ctr + 5;  // cleared at the start
   f.flags = b.ctr + 1;
   f.mode = kGenerate;
   f.increment = 0;
@@ -759,34 +759,38 @@ int refine_flow(const RefineProblem& p, int mode, const RefineIO& io, void* ws, 
     return kWorkspaceTooSmall;
   }
   const int64_t n0 = p.n - p.n_ref;
+  // the seed block of a large two-phase generate runs on the side stream beside the weights
+  // phase (which needs no seed), and so do the value buffer's fill and the FP64 table, which
+  // only the later phases read; the dataflow waits for them
+  SideStream* side = nullptr;
+  if (mode == kGenerate && p.with_seed && two_phase<T>(p, mode) && p.n_ref >= kSeedBesideMinRefined &&
+      cu_budget() == 0 && yield_cfg().dev == nullptr && seed_reserve_wgs() > 0)
+    GGP_TRY(side_stream(&side));
   {
     StageScope stage("tables", s);
-    GGP_TRY(fill_u32(b.ctr, 0u, 4, s));
+    // the counters and the dataflow's group tickets (see carve)
+    const size_t nctr = b.gtk != nullptr ? (size_t)(b.gtk + kMaxGroups * 32 - b.ctr) : 8;
+    GGP_TRY(fill_u32(b.ctr, 0u, nctr, s));
     GGP_TRY(prep_table(p.bins, p.vals, p.nb, sizeof(T) == 8, p.increment, b.tab, b.meta, s));
     if (b.tabp) GGP_TRY(prep_packed(p.bins, p.vals, p.nb, p.increment, b.tabp, s));
-    if (b.tab64) GGP_TRY(prep_table(p.bins, p.vals, p.nb, 1, 0, b.tab64, b.meta64, s));
+    if (b.tab64 && side == nullptr) GGP_TRY(prep_table(p.bins, p.vals, p.nb, 1, 0, b.tab64, b.meta64, s));
   }
   RefineArgs<T> a{};
   a.mode = mode;
   a.perm = p.perm;
   if (mode == kGenerate) {
     // graph-order buffer: seeds (given or from the dense block) then sentinels
-    // the seed block of a large two-phase generate runs on the side stream beside the
-    // weights phase (which needs no seed); the dataflow waits for it
-    SideStream* side = nullptr;
-    if (p.with_seed && two_phase<T>(p, mode) && p.n_ref >= kSeedBesideMinRefined && cu_budget() == 0 &&
-        yield_cfg().dev == nullptr && seed_reserve_wgs() > 0)
-      GGP_TRY(side_stream(&side));
     if (p.with_seed) {
-      hipLaunchKernelGGL(seed_fill_kernel<T>, dim3(grid_for(p.n, 256)), dim3(256), 0, s, b.vals, (const void*)nullptr,
-                         0, (int64_t)0, p.n);
-      GGP_LAUNCH_CHECK();
       hipStream_t ss = s;
       if (side != nullptr) {
         GGP_HIP(hipEventRecord(side->fork, s));
         GGP_HIP(hipStreamWaitEvent(side->s, side->fork, 0));
         ss = side->s;
+        GGP_TRY(prep_table(p.bins, p.vals, p.nb, 1, 0, b.tab64, b.meta64, ss));
       }
+      hipLaunchKernelGGL(seed_fill_kernel<T>, dim3(grid_for(p.n, 256)), dim3(256), 0, ss, b.vals, (const void*)nullptr,
+                         0, (int64_t)0, p.n);
+      GGP_LAUNCH_CHECK();
       StageScope stage("seed_block", ss);
       GGP_TRY(dense_factor(p.pts, p.pts_f64, p.d, n0, p.bins, p.vals, p.nb, b.dense, ss));
       hipLaunchKernelGGL(gather_seed_xi_kernel, dim3(grid_for(n0, 256)), dim3(256), 0, ss, io.xi, io.xi_f64, p.perm,

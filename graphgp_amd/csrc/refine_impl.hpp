@@ -166,6 +166,14 @@ struct RefineArgs {
 };
 
 // ------------------------------------------------------------------ helpers
+// Sets bits of an error word from a per-point error path.  An all-NaN output reaches it from
+// every point, and atomics on one address serialise at the memory (one per point made a
+// 4M-point all-NaN generate take several milliseconds more); once the bits are set, the
+// read finds them and no atomic is issued.
+__device__ __forceinline__ void raise_flag(uint32_t* f, uint32_t bits) {
+  if ((__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & bits) != bits) atomicOr(f, bits);
+}
+
 // FP32, four lanes per point: the trailing update of column COL in column MM (slot MS,
 // lane ML) for the lane's rows in slots MS..R-1, m[r][MM] -= m[r][COL] * (lane ML's
 // m[MS][COL]), as one statement of DPP-fused FMAs.  Only the column's first statement
@@ -814,7 +822,7 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
     if (mode == kLogdet) {
       double ld = 0.0;
       if (valid && owner) ld = flagged ? log(sig64) : 0.5 * log((double)skk);
-      if (valid && owner && ld != ld) atomicOr(&a.flags[0], 1u);
+      if (valid && owner && ld != ld) raise_flag(&a.flags[0], 1u);
       // block reduction in a fixed order (deterministic)
       for (int o = 32; o > 0; o >>= 1) ld += __shfl_xor(ld, o, 64);
       if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = ld;
@@ -847,7 +855,7 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
       double xi = (double)(uk / sqrt(skk));
       if (flagged && owner) xi = ((double)a.vals[fine] - mean64) / sig64;
       if (valid && owner) {
-        if (xi != xi) atomicOr(&a.flags[0], 1u);
+        if (xi != xi) raise_flag(&a.flags[0], 1u);
         const int64_t dst = (a.perm ? (int64_t)a.perm[fine] : fine) - a.io_base;
         if (a.out_f64) ((double*)a.out)[dst] = xi;
         else ((float*)a.out)[dst] = (float)xi;
@@ -876,7 +884,7 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
     }
     auto publish = [&](T v) {
       if (v != v) {
-        atomicOr(&a.flags[0], 1u);
+        raise_flag(&a.flags[0], 1u);
         v = __builtin_nan("");
       }
       __hip_atomic_store(&a.vals[fine], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1044,7 +1052,7 @@ __global__ __launch_bounds__(256) void apply_kernel(RefineArgs<T> a) {
           }
           if (owner) {
             if (v != v) {
-              atomicOr(&a.flags[0], 1u);
+              raise_flag(&a.flags[0], 1u);
               v = __builtin_nan("");
             }
             __hip_atomic_store(&a.vals[store], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
