@@ -3,7 +3,8 @@ graphgp_amd/csrc/refine_impl.hpp), run on the CPU under adversarial scheduling.
 
 What is emulated, step for step as the kernel does it:
   * persistent workgroups of 4 wavefronts x 16 points (4 lanes per point), XCD groups that
-    take blocks of 8 consecutive tiles from one counter per group, workgroup g in group
+    take blocks of consecutive tiles (32 in the kernel; 1, 2 and 32 here) from one counter
+    per group, workgroup g in group
     g % groups, a tile's ticket taken when the workgroup's previous tile is finished;
   * per lane: rows j = 4 r + lane, rows j >= k masked (never pending, zero weight and
     value), a pending bit per row; each poll round reads every pending row with a plain
@@ -28,7 +29,6 @@ import pytest
 SENT = None  # the unpublished sentinel
 P, WAVES, PPW = 4, 4, 16  # lanes per point, wavefronts per workgroup, points per wavefront
 GPT = WAVES * PPW
-GROUP_TILES = 8
 
 
 def make_graph(rng, n0, nref, k, batch):
@@ -141,7 +141,8 @@ class Workgroup:
 
     def tile_of(self, t):
         a = self.a
-        return ((t // GROUP_TILES) * a["groups"] + self.group) * GROUP_TILES + t % GROUP_TILES
+        gt = a["group_tiles"]
+        return ((t // gt) * a["groups"] + self.group) * gt + t % gt
 
     def step(self, m, out, rng):
         """Advance one random wavefront; returns False when the workgroup has exited."""
@@ -161,10 +162,10 @@ class Workgroup:
         return True
 
 
-@pytest.mark.parametrize("seed,k,batch,groups,resident,delay", [
-    (0, 5, 40, 8, 8, 3), (1, 7, 17, 3, 3, 10), (2, 30, 100, 8, 16, 5), (3, 3, 1, 2, 2, 0),
-    (4, 9, 64, 5, 40, 20), (5, 16, 1, 1, 1, 4)])
-def test_dataflow_protocol_completes_and_matches_sequential(seed, k, batch, groups, resident, delay):
+@pytest.mark.parametrize("seed,k,batch,groups,resident,delay,group_tiles", [
+    (0, 5, 40, 8, 8, 3, 2), (1, 7, 17, 3, 3, 10, 1), (2, 30, 100, 8, 16, 5, 2), (3, 3, 1, 2, 2, 0, 1),
+    (4, 9, 64, 5, 40, 20, 2), (5, 16, 1, 1, 1, 4, 32), (6, 30, 100, 8, 8, 5, 32), (7, 7, 3, 3, 4, 8, 2)])
+def test_dataflow_protocol_completes_and_matches_sequential(seed, k, batch, groups, resident, delay, group_tiles):
     rng = np.random.default_rng(seed)
     n0, nref = max(k, 8), 700
     nbrs = make_graph(rng, n0, nref, k, batch)
@@ -174,7 +175,8 @@ def test_dataflow_protocol_completes_and_matches_sequential(seed, k, batch, grou
     rm = 2 if k <= 2 * P else 4 if k <= 4 * P else 8 if k <= 8 * P else 10  # rows per lane
     ntiles = (nref + GPT - 1) // GPT
     grid = min(ntiles, 3 * groups + 1)
-    a = dict(nbrs=nbrs, w=w, sx=sx, n0=n0, nref=nref, k=k, RM=rm, ntiles=ntiles, groups=min(groups, grid))
+    a = dict(nbrs=nbrs, w=w, sx=sx, n0=n0, nref=nref, k=k, RM=rm, ntiles=ntiles, groups=min(groups, grid),
+             group_tiles=group_tiles)
     m = Memory(vals0, nref, rng, delay)
     out = {}
     counters = [0] * a["groups"]
