@@ -542,6 +542,7 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
   // variant and FP64 keep the tests: without them the compiler's schedule of their single
   // large block spills hundreds of VGPRs.
   constexpr bool GUARD = !(MODE == kWeights && FAST && sizeof(T) == 4);
+  constexpr bool LISTABLE = MODE == kWeights && sizeof(T) == 8;
   // the weights phases run in one formulation each (FP32: increments; FP64 list: plain),
   // fixed at compile time there
   const bool incr = MODE == kWeights ? sizeof(T) == 4 : a.increment != 0;
@@ -579,7 +580,9 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
     int64_t pt = tile * GPT + grp;
     bool valid = pt < a.n_ref;
     const int64_t slot_in = pt;
-    if (a.list_in) {
+    // the list form is the FP64 recomputation's (compiled out of the FP32 kernels, which then
+    // keep fewer argument registers live)
+    if (LISTABLE && a.list_in) {
       uint32_t cnt = __hip_atomic_load(a.fix_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (cnt > a.fix_cap) cnt = a.fix_cap;
       if (tile * GPT >= (int64_t)cnt) break;  // tiles are taken in order: nothing left
@@ -763,7 +766,7 @@ __global__ __launch_bounds__(256, 2) void refine_kernel(RefineArgs<T> a) {
           if (lane == il && i < k) wv[is] = wi;
         }
       });
-      if (valid && a.list_in) {
+      if (LISTABLE && valid && a.list_in) {
         // FP64 recomputation of a listed point: FP64 weights and sigma at its list slot
 #pragma unroll
         for (int r = 0; r < R; ++r)
