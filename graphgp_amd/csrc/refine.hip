@@ -683,15 +683,17 @@ int run_two_phase(RefineArgs<T> a, const RefineProblem& p, RefineWs<T>& b, hipSt
   a.pts = p.pts;
   a.nbrs = p.nbrs;
   a.nbv = p.sched_nbrs;
-  constexpr int P = 4;  // rows per lane: the smallest of 2, 4, 8, 10 covering k (k <= 40 here)
-  a.ntiles = (p.n_ref + 256 / P - 1) / (256 / P);
-  auto kern = p.k <= 2 * P ? apply_kernel<T, P, 2>
-              : p.k <= 4 * P ? apply_kernel<T, P, 4>
-              : p.k <= 8 * P ? apply_kernel<T, P, 8>
-                             : apply_kernel<T, P, 10>;
-  // four workgroups per CU: more polling workgroups only add memory traffic (eight per CU:
-  // 3.64 ms at 16.8M points, four: 3.54 ms; the poll interval made no difference)
-  const int grid = (int)std::min<int64_t>(a.ntiles, persistent_grid((const void*)kern, cus * 4, 0));
+  // Lanes per point and rows per lane: four lanes up to k = 16 (2 or 4 rows), eight lanes
+  // beyond (4 or 5 rows, k <= 40 here), with four and six workgroups per CU.  At 16.8M
+  // points, k = 30 (ms): four lanes with 3/4/6/8 workgroups per CU 3.60/3.37/3.45/3.45,
+  // eight lanes 3.58/-/3.22/3.22, sixteen lanes 4.49/-/3.73/3.58; the poll interval made no
+  // difference.
+  const bool wide = p.k > 16;
+  const int lanes = wide ? 8 : 4;
+  a.ntiles = (p.n_ref + 256 / lanes - 1) / (256 / lanes);
+  auto kern = !wide ? (p.k <= 8 ? apply_kernel<T, 4, 2> : apply_kernel<T, 4, 4>)
+                    : (p.k <= 32 ? apply_kernel<T, 8, 4> : apply_kernel<T, 8, 5>);
+  const int grid = (int)std::min<int64_t>(a.ntiles, persistent_grid((const void*)kern, cus * (wide ? 6 : 4), 0));
   a.yield = yield_cfg().dev;
   a.groups = std::min(grid, kMaxGroups);
   // every group keeps a workgroup that does not yield (the first `keep` ones do not)

@@ -2,14 +2,14 @@
 graphgp_amd/csrc/refine_impl.hpp), run on the CPU under adversarial scheduling.
 
 What is emulated, step for step as the kernel does it:
-  * persistent workgroups of 4 wavefronts x 16 points (4 lanes per point), XCD groups that
-    take blocks of consecutive tiles (32 in the kernel; 1, 2 and 32 here) from one counter
-    per group, workgroup g in group
-    g % groups, a tile's ticket taken when the workgroup's previous tile is finished;
-  * per lane: rows j = 4 r + lane, rows j >= k masked (never pending, zero weight and
+  * persistent workgroups of 4 wavefronts x 64 / P points (P = 4 lanes per point up to
+    k = 16, 8 beyond, as the launch chooses), XCD groups that take blocks of consecutive
+    tiles (32 in the kernel; 1, 2 and 32 here) from one counter per group, workgroup g in
+    group g % groups, a tile's ticket taken when the workgroup's previous tile is finished;
+  * per lane: rows j = P r + lane, rows j >= k masked (never pending, zero weight and
     value), a pending bit per row; each poll round reads every pending row with a plain
     read, and re-reads it coherently when the plain read returns the sentinel;
-  * a group (point) becomes ready when the ballot shows all 4 of its lanes with no pending
+  * a group (point) becomes ready when the ballot shows all P of its lanes with no pending
     row; it publishes once;
   * memory: a plain read of a published slot may still return the sentinel (a stale line
     in the reader's L2), a coherent read returns the sentinel until the store has become
@@ -27,8 +27,7 @@ import numpy as np
 import pytest
 
 SENT = None  # the unpublished sentinel
-P, WAVES, PPW = 4, 4, 16  # lanes per point, wavefronts per workgroup, points per wavefront
-GPT = WAVES * PPW
+WAVES = 4  # wavefronts per workgroup (of 64 lanes: 64 / P points each, P lanes per point)
 
 
 def make_graph(rng, n0, nref, k, batch):
@@ -78,8 +77,10 @@ class Memory:
 class Wave:
     def __init__(self, wg, wave, tile, m, a):
         self.points = []
-        for g in range(PPW):
-            q = tile * GPT + wave * PPW + g
+        P = a["P"]
+        ppw = 64 // P
+        for g in range(ppw):
+            q = tile * WAVES * ppw + wave * ppw + g
             valid = q < a["nref"]
             qc = q if valid else a["nref"] - 1
             lanes = []
@@ -172,10 +173,12 @@ def test_dataflow_protocol_completes_and_matches_sequential(seed, k, batch, grou
     w = rng.standard_normal((nref, k)) / k
     sx = rng.standard_normal(nref)
     vals0 = rng.standard_normal(n0)
-    rm = 2 if k <= 2 * P else 4 if k <= 4 * P else 8 if k <= 8 * P else 10  # rows per lane
-    ntiles = (nref + GPT - 1) // GPT
+    P = 8 if k > 16 else 4  # lanes per point, as the launch chooses them
+    rm = -(-k // P)  # rows per lane
+    gpt = WAVES * 64 // P
+    ntiles = (nref + gpt - 1) // gpt
     grid = min(ntiles, 3 * groups + 1)
-    a = dict(nbrs=nbrs, w=w, sx=sx, n0=n0, nref=nref, k=k, RM=rm, ntiles=ntiles, groups=min(groups, grid),
+    a = dict(nbrs=nbrs, w=w, sx=sx, n0=n0, nref=nref, k=k, P=P, RM=rm, ntiles=ntiles, groups=min(groups, grid),
              group_tiles=group_tiles)
     m = Memory(vals0, nref, rng, delay)
     out = {}
