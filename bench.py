@@ -34,7 +34,7 @@ PEAK_HBM_GBS = 8000.0
 PEAK_FP32_TFLOPS = 157.3
 # HBM bytes per launch of the fp32_increment weights kernel at N=16,777,216, k=30
 # (profiles/refine_pmc_16m.md: FETCH_SIZE x2.0 + WRITE_SIZE x1.0, copy-calibrated)
-TRAFFIC_WEIGHTS_16M_K30 = 2.5837e10
+TRAFFIC_WEIGHTS_16M_K30 = 2.5835e10
 PEAK_FP64_TFLOPS = 78.6
 
 
