@@ -84,7 +84,8 @@ enum RefineMode : int { kGenerate = 0, kInverse = 1, kLogdet = 2, kWeights = 3 }
 constexpr int kFixWaves = 2;
 
 // consecutive tiles one XCD group of the dataflow kernel takes at a time (at 16.8M points,
-// k = 30: 4 tiles 3.46 ms, 8 tiles 3.43, 16 tiles 3.39, 32 and 64 tiles 3.36)
+// k = 30, four lanes per point: 4 tiles 3.46 ms, 8 tiles 3.43, 16 tiles 3.39, 32 and 64
+// tiles 3.36; eight lanes: 16 tiles 3.24, 32 tiles 3.22, 64 tiles 3.21)
 constexpr uint32_t kGroupTiles = 32;
 constexpr int kMaxGroups = 8;  // XCDs of an MI355X
 
