@@ -598,11 +598,13 @@ int side_stream(SideStream** out) {
 }
 
 // Refined points from which the seed block of the two-phase generate runs beside the
-// weights phase, and the workgroup slots the weights phase leaves free for it meanwhile:
-// one per eight CUs (32 on MI355X; at 16.8M points, k = 30, the seed block then takes 1.5 ms
-// beside the weights phase, with 16 free slots 13.4 ms).  GGP_SEED_RESERVE overrides it;
-// 0 runs the seed block before the weights phase.
-constexpr int64_t kSeedBesideMinRefined = int64_t(1) << 20;
+// weights phase (below, the weights phase is shorter than what it would hide: at 600k
+// points, k = 30, 2.00 ms either way; at 800k 1.90 ms beside against 2.13 ms first), and
+// the workgroup slots the weights phase leaves free for it meanwhile: one per eight CUs (32
+// on MI355X; at 16.8M points the seed block then takes 1.4 ms beside the weights phase,
+// with 24 free slots 12 ms).  GGP_SEED_RESERVE overrides it; 0 runs the seed block before
+// the weights phase.
+constexpr int64_t kSeedBesideMinRefined = int64_t(3) << 18;
 int seed_reserve_wgs() {
   const char* e = getenv("GGP_SEED_RESERVE");
   return e ? atoi(e) : std::max(1, device_cus() / 8);

@@ -508,6 +508,7 @@ def test_scheduled_generate_equals_unscheduled(monkeypatch):
 
     R = sys.modules["graphgp_amd.graph"]
     p = clustered_points(200_000, d=3, seed=31)
+    monkeypatch.setattr(R, "SCHEDULE_MIN_REFINED", 1 << 40)  # built and first run unscheduled
     g = gp.build_graph(p, n0=1000, k=30)
     cov = gp.extras.matern_kernel(p=1, variance=1.0, cutoff=0.05, r_min=1e-6, r_max=2, n_bins=2001, jitter=1e-5)
     xi = torch.randn(200_000, device="cuda", generator=torch.Generator(device="cuda").manual_seed(3))
