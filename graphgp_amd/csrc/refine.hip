@@ -598,13 +598,15 @@ int side_stream(SideStream** out) {
 }
 
 // Refined points from which the seed block of the two-phase generate runs beside the
-// weights phase (below, the weights phase is shorter than what it would hide: at 600k
-// points, k = 30, 2.00 ms either way; at 800k 1.90 ms beside against 2.13 ms first), and
+// weights phase (below 600k points the weights phase is shorter than what it would hide;
+// at 800k points, k = 30, it saves 0.23 ms, but the stream is shared by the thread's calls,
+// and a served request of that size then queued behind a large request's work on it: p99
+// latency of requests up to 1e6 points 32 -> 110 ms), and
 // the workgroup slots the weights phase leaves free for it meanwhile: one per eight CUs (32
 // on MI355X; at 16.8M points the seed block then takes 1.4 ms beside the weights phase,
 // with 24 free slots 12 ms).  GGP_SEED_RESERVE overrides it; 0 runs the seed block before
 // the weights phase.
-constexpr int64_t kSeedBesideMinRefined = int64_t(3) << 18;
+constexpr int64_t kSeedBesideMinRefined = int64_t(1) << 20;
 int seed_reserve_wgs() {
   const char* e = getenv("GGP_SEED_RESERVE");
   return e ? atoi(e) : std::max(1, device_cus() / 8);
