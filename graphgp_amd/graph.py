@@ -68,7 +68,7 @@ def _offsets(depths_sorted: torch.Tensor, max_depth: int) -> Tuple[int, ...]:
 # cache lines.  It costs one sort, an (n - n0, k) int32 array and a copy of the points per
 # graph, kept on the Graph.  FP32-increment generate of the clustered 3-D mixture, k = 30,
 # with and without it (ms): 150k points 1.60 / 2.19, 300k 1.77 / 2.42, 600k 2.00 / 2.73,
-# 1.05M 2.06 / 2.87.
+# 1.05M 2.41 / 3.25.
 SCHEDULE_MIN_REFINED = 1 << 17
 
 
